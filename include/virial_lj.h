@@ -1,0 +1,210 @@
+/*
+ * virial_lj.h - C ABI of the MI355X (gfx950) Lennard-Jones library.
+ *
+ * Drop-in for the host API of virial-engine's `virial_particles::lennard_jones`
+ * (crates/virial-particles/src/lennard_jones.rs). Each entry point names the
+ * reference item it replaces. Plain C types only: the caller owns every host
+ * array and pre-sizes outputs; the library owns device memory.
+ *
+ * Errors: the reference panics (assert!/expect); here every call returns a
+ * status (VLJ_OK = 0) and vlj_last_error() holds the message, worded as the
+ * reference's panic where there is one.
+ *
+ * Threading: a handle is not thread-safe (the reference takes &mut self).
+ * Each handle owns one HIP stream on its device; vlj_step and vlj_measure
+ * only enqueue; vlj_poll never blocks; the *_now calls, vlj_run and the
+ * read calls synchronise the handle's stream.
+ *
+ * One handle holds one or more independent periodic systems ("replicas").
+ * vlj_create makes a handle with one; vlj_batch_create one with many, run by
+ * the same kernels in the same launches over a replica-offset table.
+ */
+#ifndef VIRIAL_LJ_H
+#define VIRIAL_LJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VLJ_OK 0
+#define VLJ_E_INVALID 1     /* argument the reference asserts against */
+#define VLJ_E_HIP 2         /* HIP runtime error (message in vlj_last_error) */
+#define VLJ_E_STATE 3       /* call not valid in this state (reference panics) */
+#define VLJ_E_UNSUPPORTED 4 /* feature not built */
+#define VLJ_E_NO_DEVICE 5   /* no usable gfx950 device */
+
+/* Particle (lennard_jones.rs:25-30). Mass 1, reduced units. */
+typedef struct {
+    double pos[3];
+    double vel[3];
+} vlj_particle;
+
+#define VLJ_THERMOSTAT_NONE 0
+#define VLJ_THERMOSTAT_LANGEVIN 1
+#define VLJ_THERMOSTAT_NOSE_HOOVER 2
+
+/* Thermostat (lennard_jones.rs:32-61): Langevin {temperature, friction, seed}
+ * or NoseHoover {temperature, period}. The seed keys a counter-based
+ * generator (Philox4x32-10), not the reference's per-particle PCG. */
+typedef struct {
+    uint32_t kind;
+    double temperature;
+    double friction;
+    double period;
+    uint64_t seed;
+} vlj_thermostat;
+
+#define VLJ_PRECISION_FP64 0 /* FP64 state and FP64 pair forces */
+#define VLJ_PRECISION_FP32 1 /* FP32 pair forces on FP64 displacements */
+
+/* Settings (lennard_jones.rs:63-82) plus the pair-force precision. */
+typedef struct {
+    double box_size[3];
+    double cutoff;
+    double dt;
+    double skin;
+    vlj_thermostat thermostat;
+    int32_t shifted_force;
+    uint32_t profile_bins;
+    uint32_t precision;
+} vlj_settings;
+
+/* Measurement (lennard_jones.rs:84-131). raw[] holds the 64-bit fixed-point
+ * totals the doubles were decoded from: KE, PE, W, momentum x/y/z (energy
+ * or momentum scale times 2^32), condensed count, and twice the pairs within
+ * the cutoff. */
+typedef struct {
+    uint64_t step;
+    double time;
+    double kinetic;
+    double potential;
+    double virial;
+    double momentum[3];
+    uint32_t count;
+    double volume;
+    uint32_t list_rebuilds;
+    uint32_t list_overflow;
+    double thermostat_energy;
+    uint32_t condensed;
+    uint64_t pairs_within_cutoff;
+    int64_t raw[8];
+} vlj_measurement;
+
+typedef struct vlj_sim vlj_sim;
+typedef struct vlj_sim vlj_batch;
+
+/* LennardJones::new (lennard_jones.rs:443-801): copies the particles
+ * (positions wrapped into the box), rounds the box to f32, derives cells,
+ * list capacity, fixed-point scales and pair constants. VLJ_E_INVALID with
+ * "the box must be at least three (cutoff + skin) wide on every axis" where
+ * the reference asserts (rs:458-461). No kernel runs until the first step or
+ * measurement (lazy priming, rs:1080-1089). */
+int vlj_create(const vlj_particle *particles, size_t count, const vlj_settings *settings,
+               vlj_sim **out);
+/* Drop (the reference frees its wgpu buffers on drop). */
+void vlj_destroy(vlj_sim *sim);
+
+/* LennardJones::step (rs:1092-1138): enqueue `steps` steps; returns at once. */
+int vlj_step(vlj_sim *sim, uint32_t steps);
+/* LennardJones::run (rs:1274-1284): advance and wait. */
+int vlj_run(vlj_sim *sim, uint64_t steps);
+/* LennardJones::measure (rs:1141-1163): enqueue a measurement into a free
+ * slot of the 3-slot read-back ring; *accepted = 0 when all are busy. */
+int vlj_measure(vlj_sim *sim, int32_t *accepted);
+/* LennardJones::poll (rs:1209-1246): finished measurements, oldest first;
+ * never blocks. *written <= capacity. */
+int vlj_poll(vlj_sim *sim, vlj_measurement *out, size_t capacity, size_t *written);
+/* LennardJones::measure_now (rs:1250-1260): VLJ_E_STATE with "a measurement
+ * is already in flight" when the ring is full, as the reference panics. */
+int vlj_measure_now(vlj_sim *sim, vlj_measurement *out);
+/* LennardJones::read_particles (rs:1354-1374): full precision, original order. */
+int vlj_read_particles(vlj_sim *sim, vlj_particle *out, size_t count);
+
+/* Accessors and setters (rs:870-1030). */
+uint32_t vlj_count(const vlj_sim *sim);
+int vlj_box_size(const vlj_sim *sim, double out[3]);
+double vlj_volume(const vlj_sim *sim);
+uint64_t vlj_steps(const vlj_sim *sim);
+double vlj_time(const vlj_sim *sim);
+/* target_temperature (rs:875-881): *has = 0 without a thermostat. */
+int vlj_target_temperature(const vlj_sim *sim, double *temperature, int32_t *has);
+/* set_temperature (rs:905-925): VLJ_E_STATE "set_temperature needs a
+ * thermostat" without one. */
+int vlj_set_temperature(vlj_sim *sim, double temperature);
+/* set_condensed_neighbours (rs:895-899). */
+int vlj_set_condensed_neighbours(vlj_sim *sim, uint32_t neighbours);
+
+/* Profiles along z (rs:1167-1191, 1297-1325): sums[bins * 8 + 3] are the
+ * running fixed-point sums decoded to doubles (per slab: count, m v_a^2 for
+ * a = x, y, z, W_aa, potential; then the whole-box W_aa), not yet divided by
+ * the sample count. VLJ_E_STATE "profiles need Settings::profile_bins above
+ * zero" without bins. */
+int vlj_sample_profile(vlj_sim *sim);
+int vlj_clear_profile(vlj_sim *sim);
+int vlj_read_profile(vlj_sim *sim, double *sums, size_t capacity, uint64_t *samples);
+
+/* Last error message of the handle (or of the last failed vlj_create when
+ * sim is NULL). Never NULL. */
+const char *vlj_last_error(const vlj_sim *sim);
+
+/* ---- batched replicas (no counterpart in the reference) ----
+ * `offsets[r]` .. `offsets[r + 1]` index replica r's particles in `all`
+ * (replicas + 1 entries); `settings[r]` are its settings. All replicas share
+ * the handle's launches: one kernel per phase over a replica-offset table. */
+int vlj_batch_create(const vlj_particle *all, const uint64_t *offsets,
+                     const vlj_settings *settings, size_t replicas, vlj_batch **out);
+int vlj_batch_run(vlj_batch *batch, uint64_t steps);
+/* One measurement per replica, in replica order. */
+int vlj_batch_measure_now(vlj_batch *batch, vlj_measurement *per_replica, size_t replicas);
+size_t vlj_batch_replicas(const vlj_batch *batch);
+/* Particles of every replica, concatenated as given to vlj_batch_create. */
+int vlj_batch_read_particles(vlj_batch *batch, vlj_particle *out, size_t count);
+
+/* ---- lattices (rs:1377-1441), host only ---- */
+/* lattice: `count` particles on a simple cubic lattice filling a cube. */
+int vlj_lattice(size_t count, double box_size, double temperature, uint64_t seed,
+                vlj_particle *out);
+/* lattice_box: the same in a rectangular box. */
+int vlj_lattice_box(size_t count, const double size[3], double temperature, uint64_t seed,
+                    vlj_particle *out);
+/* FCC, 4 n^3 particles at `density` (no counterpart in the reference).
+ * *side = the cube's side, already rounded to f32. */
+int vlj_fcc(uint32_t cells_per_side, double density, double temperature, uint64_t seed,
+            vlj_particle *out, double *side);
+
+/* ---- fixed point (crates/virial-precision/src/fixed.rs:9-57) ----
+ * Device reduction used by every measurement: sum of round-ties-even(values[i]
+ * * fixed_scale) as a 64-bit integer, by per-workgroup pre-reduction and one
+ * 64-bit atomic per workgroup (fixed.wgsl:71-105; KAT tests/gpu.rs:89-149). */
+int vlj_fixed_sum(const double *values, size_t count, double fixed_scale, int64_t *out);
+
+/* ---- inspection, for parity tests (no counterpart in the reference) ----
+ * State in slot order (the device keeps particles in cell order). Each
+ * pointer may be NULL. */
+int vlj_debug_state(vlj_sim *sim, double *pos, double *vel, double *acc, uint32_t *orig);
+/* Cell grid of replica 0 after the last rebuild: cell of each slot,
+ * total_cells + 1 starts (relative to the replica's first slot), slots in
+ * cell order. */
+int vlj_debug_cells(vlj_sim *sim, uint32_t *cell_of, uint32_t *cell_start, uint32_t *sorted);
+/* Neighbour lists of replica 0: counts[n] and entries[n * capacity], row per
+ * slot, relative slot indices. */
+int vlj_debug_lists(vlj_sim *sim, uint32_t *counts, uint32_t *entries);
+/* Derived integers of replica 0: cells[3], total_cells, capacity, rebuilds. */
+int vlj_debug_grid(const vlj_sim *sim, uint32_t out[6]);
+/* Derived reals of replica 0: energy_scale, momentum_scale, shift,
+ * force_shift, c1, c2. */
+int vlj_debug_params(const vlj_sim *sim, double out[6]);
+
+/* Kernel timing with HIP events on the handle's stream: runs `steps` steps
+ * launching the kernels directly (not from the captured graph) with an event
+ * pair around every pair-force launch; returns the summed force-kernel time
+ * and the whole elapsed time in milliseconds. */
+int vlj_timed_run(vlj_sim *sim, uint64_t steps, double *force_ms, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

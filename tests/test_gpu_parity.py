@@ -1,0 +1,124 @@
+"""GPU parity: the HIP library against the oracle on the same inputs.
+
+Integer work (cells, cell order, lists, fixed-point sums) must be bit-exact.
+FP64 forces, energies and trajectories follow the same IEEE operation
+sequence as the oracle, so they are compared bit for bit as well; the FP32
+pair-force form is compared with the FP64 oracle within stated tolerances.
+"""
+import numpy as np
+import pytest
+
+from helpers import jittered_box, jittered_lattice, sc_liquid, spread_values
+
+pytestmark = pytest.mark.gpu
+
+
+def make_pair(oracle, lj, pos, vel, box, cutoff=2.5, skin=0.3, dt=0.005, shifted_force=False, precision=0,
+              thermostat=None):
+    box3 = [box] * 3 if np.isscalar(box) else list(box)
+    settings = lj.Settings(box3, cutoff, dt, thermostat=thermostat, shifted_force=shifted_force, skin=skin,
+                           precision=precision)
+    sim = lj.LennardJones((pos, vel), settings)
+    kind, temp, fric, seed = 0, 0.0, 0.0, 1
+    if thermostat is not None:
+        kind, temp, fric, seed = 1, thermostat.temperature, thermostat.friction, thermostat.seed
+    ref = oracle.Sim(pos, vel, box3, cutoff=cutoff, skin=skin, dt=dt, shifted_force=shifted_force,
+                     precision=precision, thermostat=kind, temperature=temp, friction=fric, seed=seed)
+    return sim, ref
+
+
+def assert_same_grid(sim, ref):
+    g = sim.debug_grid()
+    p = ref.params
+    assert g["cells"] == list(p.cells)
+    assert g["capacity"] == p.capacity
+    cell_of, start, sorted_ = sim.debug_cells()
+    r_cell_of, r_start, r_sorted = ref.cells()
+    np.testing.assert_array_equal(cell_of, r_cell_of)
+    np.testing.assert_array_equal(start, r_start)
+    np.testing.assert_array_equal(sorted_, r_sorted)
+    counts, entries = sim.debug_lists()
+    r_counts, r_entries = ref.lists()
+    np.testing.assert_array_equal(counts, r_counts)
+    for i in range(len(counts)):
+        np.testing.assert_array_equal(entries[i, : counts[i]], r_entries[i, : r_counts[i]])
+
+
+def test_fixed_point_sum_kat(lj):
+    # tests/gpu.rs:89-149: 2^20 spread values, exact integer total, same bits every run
+    values = spread_values(1 << 20).astype(np.float64)
+    expected = sum(int(round(float(x) * 2.0 ** 32)) for x in values)
+    runs = [lj.fixed_sum(values, 2.0 ** 32) for _ in range(3)]
+    assert runs == [expected] * 3
+
+
+def test_cell_grid_finds_every_pair(oracle, lj):
+    # tests/thermodynamics.rs:55-98, at the reference's tolerance and bit for bit
+    pos, vel = jittered_lattice(oracle)
+    sim, ref = make_pair(oracle, lj, pos, vel, 11.0, dt=0.002)
+    m = sim.measure_now()
+    r = ref.measure()
+    assert_same_grid(sim, ref)
+    assert list(m.raw) == list(r.raw)
+    L = sim.box_size()[0]
+    pe, w = oracle.all_pairs(np.mod(pos, L), [L] * 3, 2.5, False)
+    assert abs((m.potential - pe) / pe) < 4e-7
+    assert abs((m.virial - w.sum()) / w.sum()) < 4e-7
+    _, _, acc, orig = sim.debug_state()
+    _, _, r_acc, r_orig = ref.state()
+    np.testing.assert_array_equal(orig, r_orig)
+    np.testing.assert_array_equal(acc, r_acc)
+
+
+@pytest.mark.parametrize("shifted_force", [False, True])
+def test_rectangular_box_matches_all_pairs(oracle, lj, shifted_force):
+    # tests/coexistence.rs:93-130 (scalar virial; the tensor comes with profiles)
+    size = [9.0, 10.5, 14.25]
+    pos, vel = jittered_box(oracle, 1000, size)
+    sim, ref = make_pair(oracle, lj, pos, vel, size, shifted_force=shifted_force)
+    m = sim.measure_now()
+    r = ref.measure()
+    assert_same_grid(sim, ref)
+    assert list(m.raw) == list(r.raw)
+    box = sim.box_size()
+    wrapped = np.stack([np.mod(pos[:, k], box[k]) for k in range(3)], axis=1)
+    pe, w = oracle.all_pairs(wrapped, box, 2.5, shifted_force)
+    assert abs((m.potential - pe) / pe) < 4e-7
+    assert abs((m.virial - w.sum()) / w.sum()) < 4e-7
+
+
+def test_nve_trajectory_bit_exact(oracle, lj):
+    # SC 10^3 at rho 0.8442, T 0.722, rc 2.5 truncated-shifted, dt 0.005
+    pos, vel, box = sc_liquid(oracle)
+    sim, ref = make_pair(oracle, lj, pos, vel, box)
+    for steps in (1, 9, 90):
+        sim.run(steps)
+        ref.run(steps)
+        m, r = sim.measure_now(), ref.measure()
+        assert list(m.raw) == list(r.raw), f"after {sim.steps()} steps"
+        assert m.list_rebuilds == r.list_rebuilds
+    p, v, a, o = sim.debug_state()
+    rp, rv, ra, ro = ref.state()
+    np.testing.assert_array_equal(o, ro)
+    np.testing.assert_array_equal(p, rp)
+    np.testing.assert_array_equal(v, rv)
+    np.testing.assert_array_equal(a, ra)
+    assert_same_grid(sim, ref)
+
+
+def test_fp32_forces_against_fp64_oracle(oracle, lj):
+    pos, vel, box = sc_liquid(oracle, n_side=12)
+    pos = oracle.jitter(pos, 99, 0.3)
+    sim, _ = make_pair(oracle, lj, pos, vel, box, precision=1)
+    ref = oracle.Sim(pos, vel, [box] * 3, precision=0)
+    m, r = sim.measure_now(), ref.measure()
+    _, _, acc, orig = sim.debug_state()
+    _, _, r_acc, r_orig = ref.state()
+    np.testing.assert_array_equal(orig, r_orig)
+    err = np.linalg.norm(acc - r_acc, axis=1)
+    rel_rms = np.sqrt(np.mean(err ** 2) / np.mean(np.sum(r_acc ** 2, axis=1)))
+    print(f"FP32 forces: rms relative error {rel_rms:.2e}; PE {(m.potential - r.potential) / r.potential:.2e}; "
+          f"W {(m.virial - r.virial) / r.virial:.2e}")
+    assert rel_rms < 1e-6
+    assert abs((m.potential - r.potential) / r.potential) < 4e-7
+    assert abs((m.virial - r.virial) / r.virial) < 4e-7

@@ -1,0 +1,227 @@
+// Device-side layout and arithmetic of the Lennard-Jones path.
+//
+// One handle holds one or more independent periodic systems ("replicas").
+// Every per-atom array is one flat FP64/u32 array over all replicas, replica r
+// owning atoms [atom_off, atom_off + count) in slot order (slots are kept in
+// cell order, lennard_jones.wgsl:31-33). Launches cover all replicas at once:
+// each workgroup reads a Range (replica, first, end) from a table, so a block
+// never straddles two replicas and the replica's parameters stay uniform
+// (scalar registers) inside it.
+//
+// All kernels are built with -ffp-contract=off; every fused multiply-add is
+// written as fma() so the operation sequence is fixed and reproducible.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vlj {
+
+constexpr int kBlock = 256;       // threads per atom block (4 wave64)
+constexpr int kScanCells = 1024;  // cells per scan block (256 threads x 4)
+constexpr int kSortCells = 4;     // cells per sort block (one wave64 per cell)
+constexpr int kSlots = 8;         // fixed-point slots per replica
+constexpr double kNearR2 = 2.25;  // condensed: neighbours within 1.5 sigma (lennard_jones.wgsl:417)
+
+// Parameters of one replica (lennard_jones.rs:282-313, :443-560).
+struct Replica {
+  double box[3];
+  double half_box[3];
+  double inv_box[3];
+  float cell_size[3];
+  uint32_t cells[3];
+  uint32_t atom_off, count;
+  uint32_t cell_off, total_cells;
+  uint32_t scan_block_off, scan_blocks;
+  uint32_t capacity;
+  uint32_t thermostat;  // 0 none, 1 Langevin (BAOAB)
+  uint64_t list_off;    // first neighbour entry of the replica
+  double rc, rc2, list_r2, half_skin2;
+  double shift, force_shift;
+  double dt, half_dt, c1, c2;
+  double energy_fixed, momentum_fixed;  // scale * 2^32
+  uint64_t seed;
+  uint32_t near_count;
+  uint32_t index;
+};
+
+// Control words of one replica (lennard_jones.wgsl:106-110: MAX_DISPLACEMENT,
+// REBUILDS, OVERFLOW, NEVER_BUILT), plus the step counter of the noise.
+struct Control {
+  unsigned long long max_moved2;  // FP64 bits; non-negative doubles order as integers
+  unsigned long long step;        // steps taken
+  uint32_t rebuild;               // this step rebuilds the lists
+  uint32_t reorder;               // ... and moves atoms into cell order
+  uint32_t rebuilds;
+  uint32_t overflow;
+  uint32_t built;
+  uint32_t pad;
+};
+
+struct Range {
+  uint32_t rep, first, end, pad;
+};
+
+// Everything a kernel touches, passed by value.
+struct Buffers {
+  const Replica* rep;
+  Control* ctl;
+  const Range* atom_blocks;
+  const Range* scan_blocks;
+  const Range* sort_blocks;
+  uint32_t n_atom_blocks, n_scan_blocks, n_sort_blocks, n_replicas;
+  double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;           // state, slot order
+  double *x2, *y2, *z2, *vx2, *vy2, *vz2, *ax2, *ay2, *az2;  // reorder target
+  double *xb, *yb, *zb;                                      // position at last list build
+  uint32_t *orig, *orig2;                                    // original index of each slot
+  uint32_t *cell_of, *cell_of2;                              // global cell of each slot
+  uint32_t *unsorted, *sorted;                               // slots in cell order
+  uint32_t *cell_count, *cell_cursor, *cell_scan, *cell_start, *cell_n;
+  uint32_t *bsum, *bprefix;                                  // per scan block
+  uint32_t *nbr_count, *nbr;                                 // transposed lists
+  long long* slots;                                          // kSlots per replica
+  uint32_t* tickets;                                         // last-block counters
+};
+
+// ---- R1: nearest periodic image. The image shift k = rint(d / L) is
+// taken off with one rounding, fma(-k, L, d) = d - kL; kL is exact (L has
+// 24 significant bits), so this equals the compare form d > L/2 -> d - L,
+// d < -L/2 -> d + L for every |d| away from L/2 (lennard_jones.wgsl:198-202).
+__device__ __forceinline__ double nearest(double d, double side, double inv) {
+  double k = __builtin_rint(d * inv);
+  return __builtin_fma(-k, side, d);
+}
+
+__device__ __forceinline__ double dist2(double dx, double dy, double dz) {
+  return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+}
+
+// ---- R2: cell of a position, f32 rule of grid.wgsl:11-20.
+__device__ __forceinline__ uint32_t cell_axis(double x, float cs, uint32_t n) {
+  float q = (float)x / cs;  // IEEE f32 division (correctly rounded by default)
+  int c = (int)floorf(q);
+  int m = (int)n - 1;
+  c = c < 0 ? 0 : (c > m ? m : c);
+  return (uint32_t)c;
+}
+
+__device__ __forceinline__ uint32_t cell_local(const Replica& R, double x, double y, double z) {
+  uint32_t cx = cell_axis(x, R.cell_size[0], R.cells[0]);
+  uint32_t cy = cell_axis(y, R.cell_size[1], R.cells[1]);
+  uint32_t cz = cell_axis(z, R.cell_size[2], R.cells[2]);
+  return (cz * R.cells[1] + cy) * R.cells[0] + cx;
+}
+
+// ---- Counter-based noise: Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int round = 0; round < 10; round++) {
+    uint32_t lo0 = 0xD2511F53u * c[0];
+    uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+    uint32_t lo1 = 0xCD9E8D57u * c[2];
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    uint32_t n0 = hi1 ^ c[1] ^ k0;
+    uint32_t n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Natural log from +, *, fma and / only (range reduction to [sqrt(1/2),
+// sqrt(2)), 2 atanh((m-1)/(m+1)) by its odd series, ln 2 split in two after
+// Cody and Waite). Positive normal arguments.
+__device__ __forceinline__ double log_basic(double x) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  int e = (int)((b >> 52) & 0x7ff) - 1023;
+  b = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+  double m = __longlong_as_double((long long)b);
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    e += 1;
+  }
+  double f = m - 1.0;
+  double s = f / (2.0 + f);
+  double z = s * s;
+  double p = 1.0 / 21.0;
+  p = __builtin_fma(p, z, 1.0 / 19.0);
+  p = __builtin_fma(p, z, 1.0 / 17.0);
+  p = __builtin_fma(p, z, 1.0 / 15.0);
+  p = __builtin_fma(p, z, 1.0 / 13.0);
+  p = __builtin_fma(p, z, 1.0 / 11.0);
+  p = __builtin_fma(p, z, 1.0 / 9.0);
+  p = __builtin_fma(p, z, 1.0 / 7.0);
+  p = __builtin_fma(p, z, 1.0 / 5.0);
+  p = __builtin_fma(p, z, 1.0 / 3.0);
+  double tail = 2.0 * (s * z * p);
+  double de = (double)e;
+  const double ln2_hi = 6.93147180369123816490e-01;
+  const double ln2_lo = 1.90821492927058770002e-10;
+  return __builtin_fma(de, ln2_hi, 2.0 * s + __builtin_fma(de, ln2_lo, tail));
+}
+
+// sin and cos of 2 pi u, u in [0, 1]: exact removal of the nearest quarter
+// turn, Taylor series on [-pi/4, pi/4], quadrant rotation.
+__device__ __forceinline__ void sincos_turns(double u, double* s_out, double* c_out) {
+  double q = __builtin_rint(4.0 * u);
+  double t = u - 0.25 * q;
+  double x = t * 6.283185307179586;
+  double z = x * x;
+  double sp = -1.0 / 355687428096000.0;
+  sp = __builtin_fma(sp, z, 1.0 / 1307674368000.0);
+  sp = __builtin_fma(sp, z, -1.0 / 6227020800.0);
+  sp = __builtin_fma(sp, z, 1.0 / 39916800.0);
+  sp = __builtin_fma(sp, z, -1.0 / 362880.0);
+  sp = __builtin_fma(sp, z, 1.0 / 5040.0);
+  sp = __builtin_fma(sp, z, -1.0 / 120.0);
+  sp = __builtin_fma(sp, z, 1.0 / 6.0);
+  double sn = __builtin_fma(-x * z, sp, x);
+  double cp = 1.0 / 6402373705728000.0;
+  cp = __builtin_fma(cp, z, -1.0 / 20922789888000.0);
+  cp = __builtin_fma(cp, z, 1.0 / 87178291200.0);
+  cp = __builtin_fma(cp, z, -1.0 / 479001600.0);
+  cp = __builtin_fma(cp, z, 1.0 / 3628800.0);
+  cp = __builtin_fma(cp, z, -1.0 / 40320.0);
+  cp = __builtin_fma(cp, z, 1.0 / 720.0);
+  cp = __builtin_fma(cp, z, -1.0 / 24.0);
+  cp = __builtin_fma(cp, z, 0.5);
+  double cs = __builtin_fma(-z, cp, 1.0);
+  int quadrant = ((int)q) & 3;
+  double so = quadrant == 0 ? sn : (quadrant == 1 ? cs : (quadrant == 2 ? -sn : -cs));
+  double co = quadrant == 0 ? cs : (quadrant == 1 ? -sn : (quadrant == 2 ? -cs : sn));
+  *s_out = so;
+  *c_out = co;
+}
+
+// Three standard normals for (seed, replica, original atom, step): counter
+// (step lo, step hi, atom, replica), key (seed lo, seed hi); uniforms
+// (w + 0.5) 2^-32; Box-Muller on (u0, u1) and the cosine branch on (u2, u3).
+__device__ __forceinline__ void langevin_noise(uint64_t seed, uint32_t replica, uint32_t atom,
+                                               unsigned long long step, double n[3]) {
+  uint32_t c[4] = {(uint32_t)step, (uint32_t)(step >> 32), atom, replica};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  double u0 = ((double)c[0] + 0.5) * 2.3283064365386963e-10;
+  double u1 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
+  double u2 = ((double)c[2] + 0.5) * 2.3283064365386963e-10;
+  double u3 = ((double)c[3] + 0.5) * 2.3283064365386963e-10;
+  double r0 = __builtin_sqrt(-2.0 * log_basic(u0));
+  double s0, c0;
+  sincos_turns(u1, &s0, &c0);
+  double r1 = __builtin_sqrt(-2.0 * log_basic(u2));
+  double s1, c1;
+  sincos_turns(u3, &s1, &c1);
+  n[0] = r0 * c0;
+  n[1] = r0 * s0;
+  n[2] = r1 * c1;
+}
+
+// Round-ties-even of x * fixed_scale as a 64-bit integer (fixed.wgsl:16-55
+// in FP64: the scale is a power of two, so the product is exact).
+__device__ __forceinline__ long long to_fixed(double x, double fixed_scale) {
+  return (long long)__builtin_rint(x * fixed_scale);
+}
+
+}  // namespace vlj

@@ -1,0 +1,18 @@
+// Host-side launchers of the kernels in vlj_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "vlj_device.h"
+
+namespace vlj {
+
+hipError_t launch_decide(const Buffers& B, hipStream_t s);
+hipError_t launch_drift(const Buffers& B, bool langevin, hipStream_t s);
+hipError_t launch_rebuild(const Buffers& B, hipStream_t s);
+hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s);
+hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStream_t s);
+hipError_t launch_fixed_sum(const double* values, size_t n, double scale, long long* out,
+                            hipStream_t s);
+
+}  // namespace vlj

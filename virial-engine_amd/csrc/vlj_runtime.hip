@@ -1,0 +1,984 @@
+// Host runtime and C ABI of the Lennard-Jones library (include/virial_lj.h).
+//
+// The handle owns one HIP stream, every device buffer, a 3-slot pinned
+// read-back ring for measurements and the step sequence captured as
+// hipGraphs (one step and kGraphSteps steps). It replaces the wgpu-side host
+// of crates/virial-particles/src/lennard_jones.rs:443-1375.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/virial_lj.h"
+#include "vlj_device.h"
+#include "vlj_launch.h"
+
+using namespace vlj;
+
+namespace {
+
+constexpr int kRing = 3;         // read-back slots (lennard_jones.rs:320)
+constexpr uint32_t kGraphSteps = 16;
+constexpr double kTwo32 = 4294967296.0;
+
+thread_local std::string g_error = "no error";
+
+struct Ring {
+  long long* slots = nullptr;  // pinned
+  Control* ctl = nullptr;      // pinned
+  hipEvent_t done = nullptr;
+  uint64_t step = 0;
+  bool in_flight = false;
+};
+
+struct HostReplica {
+  vlj_settings settings;
+  double energy_scale, momentum_scale;
+};
+
+}  // namespace
+
+struct vlj_sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error = "no error";
+  std::vector<Replica> rep;
+  std::vector<HostReplica> host;
+  std::vector<Range> atom_blocks, scan_blocks, sort_blocks;
+  uint32_t n_total = 0, cells_total = 0;
+  uint64_t list_total = 0;
+  int precision = 0;
+  bool shifted = false;
+  bool any_langevin = false;
+  Buffers B{};
+  std::vector<void*> allocations;
+  Ring ring[kRing];
+  hipGraphExec_t graph1 = nullptr, graphN = nullptr;
+  bool primed = false;
+  uint64_t steps = 0;
+  long long* fixed_scratch = nullptr;
+};
+
+namespace {
+
+int fail(vlj_sim* s, int code, const std::string& msg) {
+  if (s) s->error = msg;
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(sim, expr)                                                             \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail((sim), VLJ_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// lennard_jones.rs:435-440
+double scale_for(double magnitude) {
+  if (magnitude <= 0.0 || !std::isfinite(magnitude)) return 1.0;
+  return std::ldexp(1.0, (int)std::floor(20.0 - std::log2(magnitude)));
+}
+
+double wrap_initial(double x, double side) {  // Rust rem_euclid (lennard_jones.rs:477)
+  double r = std::fmod(x, side);
+  return r < 0.0 ? r + side : r;
+}
+
+uint32_t host_cell_axis(double x, float cs, uint32_t n) {
+  float q = (float)x / cs;
+  int c = (int)std::floor(q);
+  int m = (int)n - 1;
+  return (uint32_t)(c < 0 ? 0 : (c > m ? m : c));
+}
+
+// lennard_jones.rs:443-560 for one replica. Deliberate changes: list
+// capacity from the densest cell of the initial state (not only the box
+// average), and dt kept in FP64.
+int derive(const vlj_particle* p, size_t n, const vlj_settings& st, Replica& R, HostReplica& H,
+           std::string& err) {
+  std::memset(&R, 0, sizeof(R));
+  H.settings = st;
+  if (!(st.cutoff > 0.0) || !(st.skin >= 0.0) || !(st.dt > 0.0)) {
+    err = "cutoff and dt must be positive and skin non-negative";
+    return VLJ_E_INVALID;
+  }
+  if (st.thermostat.kind > VLJ_THERMOSTAT_NOSE_HOOVER) {
+    err = "unknown thermostat";
+    return VLJ_E_INVALID;
+  }
+  if (st.thermostat.kind == VLJ_THERMOSTAT_NOSE_HOOVER) {
+    err = "the Nose-Hoover thermostat is not built yet";
+    return VLJ_E_UNSUPPORTED;
+  }
+  if (st.precision > VLJ_PRECISION_FP32) {
+    err = "unknown precision";
+    return VLJ_E_INVALID;
+  }
+  const double list_radius = st.cutoff + st.skin;
+  uint32_t cells[3];
+  for (int k = 0; k < 3; k++) {
+    if (!(st.box_size[k] > 0.0) || !std::isfinite(st.box_size[k])) {
+      err = "box sides must be positive";
+      return VLJ_E_INVALID;
+    }
+    R.box[k] = (double)(float)st.box_size[k];
+    R.half_box[k] = 0.5 * R.box[k];
+    R.inv_box[k] = 1.0 / R.box[k];
+    double c = std::floor(R.box[k] / list_radius);
+    cells[k] = c < 0 ? 0 : (c > 4.0e9 ? 4000000000u : (uint32_t)c);
+  }
+  for (int k = 0; k < 3; k++)
+    if (cells[k] < 3) {
+      err = "the box must be at least three (cutoff + skin) wide on every axis";
+      return VLJ_E_INVALID;
+    }
+  while ((uint64_t)cells[0] * cells[1] * cells[2] > (1u << 20)) {
+    int longest = 0;
+    for (int k = 1; k < 3; k++)
+      if (cells[k] >= cells[longest]) longest = k;
+    cells[longest] -= 1;
+  }
+  double cell_size[3];
+  for (int k = 0; k < 3; k++) {
+    R.cells[k] = cells[k];
+    cell_size[k] = R.box[k] / (double)cells[k];
+    R.cell_size[k] = (float)cell_size[k];
+  }
+  R.total_cells = cells[0] * cells[1] * cells[2];
+  R.count = (uint32_t)n;
+  const double volume = R.box[0] * R.box[1] * R.box[2];
+  const double density = (double)n / volume;
+  const double sphere = 4.0 / 3.0 * 3.14159265358979323846 * list_radius * list_radius * list_radius;
+  std::vector<uint32_t> occ(R.total_cells, 0);
+  uint32_t occ_max = 0;
+  for (size_t i = 0; i < n; i++) {
+    double w[3];
+    for (int k = 0; k < 3; k++) {
+      if (!std::isfinite(p[i].pos[k]) || !std::isfinite(p[i].vel[k])) {
+        err = "particle positions and velocities must be finite";
+        return VLJ_E_INVALID;
+      }
+      w[k] = wrap_initial(p[i].pos[k], R.box[k]);
+    }
+    uint32_t c = (host_cell_axis(w[2], R.cell_size[2], cells[2]) * cells[1] +
+                  host_cell_axis(w[1], R.cell_size[1], cells[1])) *
+                     cells[0] +
+                 host_cell_axis(w[0], R.cell_size[0], cells[0]);
+    occ_max = std::max(occ_max, ++occ[c]);
+  }
+  const double cell_volume = cell_size[0] * cell_size[1] * cell_size[2];
+  const double use = std::max((double)occ_max / cell_volume, density);
+  R.capacity = (uint32_t)std::ceil(2.0 * sphere * use + 32.0);
+
+  double kinetic = 0.0, speed = 0.0;
+  for (size_t i = 0; i < n; i++) {
+    double v2 = 0.0;
+    for (int k = 0; k < 3; k++) v2 += p[i].vel[k] * p[i].vel[k];
+    kinetic += 0.5 * v2;
+    speed += std::sqrt(v2);
+  }
+  const bool langevin = st.thermostat.kind == VLJ_THERMOSTAT_LANGEVIN;
+  const double thermal = langevin ? 1.5 * st.thermostat.temperature * (double)n : 0.0;
+  H.energy_scale = scale_for(std::max(std::max(kinetic, thermal), (double)n));
+  H.momentum_scale = scale_for(std::max(speed, (double)n));
+  R.energy_fixed = H.energy_scale * kTwo32;
+  R.momentum_fixed = H.momentum_scale * kTwo32;
+  R.rc = st.cutoff;
+  R.rc2 = st.cutoff * st.cutoff;
+  R.list_r2 = list_radius * list_radius;
+  R.half_skin2 = 0.25 * st.skin * st.skin;
+  const double inv6 = 1.0 / (R.rc2 * R.rc2 * R.rc2);
+  R.shift = 4.0 * (inv6 * inv6 - inv6);
+  R.force_shift = st.shifted_force ? 24.0 * (2.0 * inv6 * inv6 - inv6) / st.cutoff : 0.0;
+  R.dt = st.dt;
+  R.half_dt = 0.5 * st.dt;
+  R.thermostat = langevin ? 1u : 0u;
+  R.c1 = 1.0;
+  R.c2 = 0.0;
+  if (langevin) {
+    if (!(st.thermostat.temperature >= 0.0) || !(st.thermostat.friction >= 0.0)) {
+      err = "thermostat temperature and friction must be non-negative";
+      return VLJ_E_INVALID;
+    }
+    R.c1 = std::exp(-st.thermostat.friction * st.dt);
+    R.c2 = std::sqrt((1.0 - R.c1 * R.c1) * st.thermostat.temperature);
+  }
+  R.seed = st.thermostat.seed;
+  R.near_count = 4;
+  return VLJ_OK;
+}
+
+template <typename T>
+int dev_alloc(vlj_sim* s, T** out, size_t count) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess)
+    return fail(s, VLJ_E_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  s->allocations.push_back(p);
+  *out = (T*)p;
+  return VLJ_OK;
+}
+
+void free_all(vlj_sim* s) {
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->graph1) (void)hipGraphExecDestroy(s->graph1);
+  if (s->graphN) (void)hipGraphExecDestroy(s->graphN);
+  for (auto& r : s->ring) {
+    if (r.slots) (void)hipHostFree(r.slots);
+    if (r.ctl) (void)hipHostFree(r.ctl);
+    if (r.done) (void)hipEventDestroy(r.done);
+  }
+  for (void* p : s->allocations) (void)hipFree(p);
+  s->allocations.clear();
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  s->stream = nullptr;
+}
+
+int check_device(vlj_sim* s) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return fail(s, VLJ_E_NO_DEVICE, "no HIP device available (this library runs on gfx950)");
+  HIP_TRY(s, hipGetDevice(&s->device));
+  hipDeviceProp_t prop;
+  HIP_TRY(s, hipGetDeviceProperties(&prop, s->device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(s, VLJ_E_NO_DEVICE,
+                std::string("this library is built for gfx950; the device is ") + prop.gcnArchName);
+  return VLJ_OK;
+}
+
+// Enqueue one step: drift, rebuild chain, forces + kick.
+int enqueue_step(vlj_sim* s) {
+  HIP_TRY(s, launch_drift(s->B, s->any_langevin, s->stream));
+  HIP_TRY(s, launch_rebuild(s->B, s->stream));
+  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, true, s->stream));
+  return VLJ_OK;
+}
+
+int capture(vlj_sim* s, uint32_t steps, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIP_TRY(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = VLJ_OK;
+  for (uint32_t t = 0; t < steps && rc == VLJ_OK; t++) rc = enqueue_step(s);
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (rc != VLJ_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(s, VLJ_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiateWithFlags(out, g, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return fail(s, VLJ_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return VLJ_OK;
+}
+
+// lennard_jones.rs:1080-1089: first lists and accelerations.
+int prime_if_needed(vlj_sim* s) {
+  if (s->primed) return VLJ_OK;
+  HIP_TRY(s, launch_decide(s->B, s->stream));
+  HIP_TRY(s, launch_rebuild(s->B, s->stream));
+  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, false, s->stream));
+  s->primed = true;
+  return VLJ_OK;
+}
+
+// Builds a handle over `replicas` systems.
+int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings* settings,
+           size_t replicas, vlj_sim** out) {
+  if (!out) return fail(nullptr, VLJ_E_INVALID, "out must not be NULL");
+  *out = nullptr;
+  if (replicas == 0 || !settings || !offsets)
+    return fail(nullptr, VLJ_E_INVALID, "at least one replica with settings is needed");
+  uint64_t total = offsets[replicas];
+  if (offsets[0] != 0) return fail(nullptr, VLJ_E_INVALID, "offsets must start at 0");
+  for (size_t r = 0; r < replicas; r++)
+    if (offsets[r + 1] < offsets[r]) return fail(nullptr, VLJ_E_INVALID, "offsets must not decrease");
+  if (total >= (1ull << 31)) return fail(nullptr, VLJ_E_INVALID, "at most 2^31 - 1 particles");
+  if (total && !all) return fail(nullptr, VLJ_E_INVALID, "particles must not be NULL");
+  for (size_t r = 1; r < replicas; r++)
+    if (settings[r].precision != settings[0].precision ||
+        (settings[r].shifted_force != 0) != (settings[0].shifted_force != 0))
+      return fail(nullptr, VLJ_E_INVALID,
+                  "replicas of one batch share the pair-force precision and cutoff style");
+
+  vlj_sim* s = new vlj_sim();
+  int rc = VLJ_OK;
+  s->rep.resize(replicas);
+  s->host.resize(replicas);
+  s->precision = (int)settings[0].precision;
+  s->shifted = settings[0].shifted_force != 0;
+  uint32_t atom_off = 0, cell_off = 0, scan_off = 0;
+  uint64_t list_off = 0;
+  for (size_t r = 0; r < replicas; r++) {
+    Replica& R = s->rep[r];
+    std::string err;
+    size_t n = (size_t)(offsets[r + 1] - offsets[r]);
+    rc = derive(all + offsets[r], n, settings[r], R, s->host[r], err);
+    if (rc != VLJ_OK) {
+      delete s;
+      return fail(nullptr, rc, replicas > 1 ? "replica " + std::to_string(r) + ": " + err : err);
+    }
+    R.index = (uint32_t)r;
+    R.atom_off = atom_off;
+    R.cell_off = cell_off;
+    R.scan_block_off = scan_off;
+    R.scan_blocks = (R.total_cells + kScanCells - 1) / kScanCells;
+    R.list_off = list_off;
+    for (uint32_t b = 0; b < R.count; b += kBlock)
+      s->atom_blocks.push_back({(uint32_t)r, atom_off + b, atom_off + std::min(R.count, b + kBlock), 0});
+    for (uint32_t b = 0; b < R.total_cells; b += kScanCells)
+      s->scan_blocks.push_back(
+          {(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + kScanCells), 0});
+    for (uint32_t b = 0; b < R.total_cells; b += kSortCells)
+      s->sort_blocks.push_back(
+          {(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + kSortCells), 0});
+    atom_off += R.count;
+    cell_off += R.total_cells;
+    scan_off += R.scan_blocks;
+    list_off += (uint64_t)R.capacity * R.count;
+    if (R.thermostat == 1) s->any_langevin = true;
+  }
+  s->n_total = atom_off;
+  s->cells_total = cell_off;
+  s->list_total = list_off;
+  if (list_off >= (1ull << 40)) {
+    delete s;
+    return fail(nullptr, VLJ_E_INVALID, "neighbour lists would not fit in device memory");
+  }
+  // Arguments are validated first, so their errors do not depend on a device.
+  rc = check_device(s);
+  if (rc != VLJ_OK) {
+    std::string msg = s->error;
+    delete s;
+    return fail(nullptr, rc, msg);
+  }
+
+  rc = VLJ_OK;
+  auto A = [&](auto** p, size_t count) {
+    if (rc == VLJ_OK) rc = dev_alloc(s, p, count);
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return fail(nullptr, VLJ_E_HIP, "hipStreamCreate failed");
+  }
+  Buffers& B = s->B;
+  const size_t N = s->n_total, C = s->cells_total;
+  Replica* rep_d = nullptr;
+  Range *ab = nullptr, *sb = nullptr, *ob = nullptr;
+  A(&rep_d, replicas);
+  A(&B.ctl, replicas);
+  A(&ab, s->atom_blocks.size());
+  A(&sb, s->scan_blocks.size());
+  A(&ob, s->sort_blocks.size());
+  double** d[] = {&B.x, &B.y, &B.z, &B.vx, &B.vy, &B.vz, &B.ax, &B.ay, &B.az,
+                  &B.x2, &B.y2, &B.z2, &B.vx2, &B.vy2, &B.vz2, &B.ax2, &B.ay2, &B.az2,
+                  &B.xb, &B.yb, &B.zb};
+  for (double** p : d) A(p, N);
+  uint32_t** u[] = {&B.orig, &B.orig2, &B.cell_of, &B.cell_of2, &B.unsorted, &B.sorted, &B.nbr_count};
+  for (uint32_t** p : u) A(p, N);
+  uint32_t** c[] = {&B.cell_count, &B.cell_cursor, &B.cell_scan, &B.cell_start, &B.cell_n};
+  for (uint32_t** p : c) A(p, C);
+  A(&B.bsum, s->scan_blocks.size());
+  A(&B.bprefix, s->scan_blocks.size());
+  A(&B.nbr, s->list_total);
+  A(&B.slots, replicas * kSlots);
+  A(&B.tickets, 4);
+  A(&s->fixed_scratch, kSlots);
+  if (rc != VLJ_OK) {
+    std::string msg = s->error;
+    free_all(s);
+    delete s;
+    return fail(nullptr, rc, msg);
+  }
+  B.rep = rep_d;
+  B.atom_blocks = ab;
+  B.scan_blocks = sb;
+  B.sort_blocks = ob;
+  B.n_atom_blocks = (uint32_t)s->atom_blocks.size();
+  B.n_scan_blocks = (uint32_t)s->scan_blocks.size();
+  B.n_sort_blocks = (uint32_t)s->sort_blocks.size();
+  B.n_replicas = (uint32_t)replicas;
+
+  // Host-side initial state: wrapped positions, slot = original index.
+  std::vector<double> h(N);
+  std::vector<uint32_t> hu(N);
+  auto upload = [&](double* dst) { return hipMemcpy(dst, h.data(), N * sizeof(double), hipMemcpyHostToDevice); };
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 3 && e == hipSuccess; k++) {
+    for (size_t r = 0; r < replicas; r++)
+      for (uint32_t i = 0; i < s->rep[r].count; i++)
+        h[s->rep[r].atom_off + i] = wrap_initial(all[offsets[r] + i].pos[k], s->rep[r].box[k]);
+    e = upload(k == 0 ? B.x : (k == 1 ? B.y : B.z));
+    if (e != hipSuccess) break;
+    for (size_t r = 0; r < replicas; r++)
+      for (uint32_t i = 0; i < s->rep[r].count; i++) h[s->rep[r].atom_off + i] = all[offsets[r] + i].vel[k];
+    e = upload(k == 0 ? B.vx : (k == 1 ? B.vy : B.vz));
+  }
+  for (size_t r = 0; r < replicas; r++)
+    for (uint32_t i = 0; i < s->rep[r].count; i++) hu[s->rep[r].atom_off + i] = i;
+  std::vector<Control> ctl(replicas);
+  std::memset(ctl.data(), 0, ctl.size() * sizeof(Control));
+  if (e == hipSuccess) e = hipMemcpy(B.orig, hu.data(), N * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(rep_d, s->rep.data(), replicas * sizeof(Replica), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(B.ctl, ctl.data(), replicas * sizeof(Control), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ab, s->atom_blocks.data(), s->atom_blocks.size() * sizeof(Range), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(sb, s->scan_blocks.data(), s->scan_blocks.size() * sizeof(Range), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ob, s->sort_blocks.data(), s->sort_blocks.size() * sizeof(Range), hipMemcpyHostToDevice);
+  // Zeroed once here; afterwards the sort kernel leaves them zero.
+  if (e == hipSuccess) e = hipMemset(B.cell_count, 0, std::max<size_t>(C, 1) * 4);
+  if (e == hipSuccess) e = hipMemset(B.cell_cursor, 0, std::max<size_t>(C, 1) * 4);
+  if (e == hipSuccess) e = hipMemset(B.tickets, 0, 16);
+  if (e == hipSuccess) e = hipMemset(B.ax, 0, std::max<size_t>(N, 1) * 8);
+  if (e == hipSuccess) e = hipMemset(B.ay, 0, std::max<size_t>(N, 1) * 8);
+  if (e == hipSuccess) e = hipMemset(B.az, 0, std::max<size_t>(N, 1) * 8);
+  if (e == hipSuccess) e = hipMemset(B.nbr_count, 0, std::max<size_t>(N, 1) * 4);
+  for (auto& r : s->ring) {
+    if (e == hipSuccess) e = hipHostMalloc((void**)&r.slots, replicas * kSlots * sizeof(long long), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&r.ctl, replicas * sizeof(Control), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r.done, hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    std::string msg = std::string("initial upload: ") + hipGetErrorString(e);
+    free_all(s);
+    delete s;
+    return fail(nullptr, VLJ_E_HIP, msg);
+  }
+  *out = s;
+  return VLJ_OK;
+}
+
+int ensure_graphs(vlj_sim* s) {
+  if (s->graph1) return VLJ_OK;
+  int rc = capture(s, 1, &s->graph1);
+  if (rc == VLJ_OK) rc = capture(s, kGraphSteps, &s->graphN);
+  return rc;
+}
+
+int enqueue_steps(vlj_sim* s, uint64_t steps) {
+  int rc = prime_if_needed(s);
+  if (rc != VLJ_OK) return rc;
+  if (steps == 0 || s->n_total == 0) {
+    s->steps += steps;
+    return VLJ_OK;
+  }
+  rc = ensure_graphs(s);
+  if (rc != VLJ_OK) return rc;
+  uint64_t left = steps;
+  while (left >= kGraphSteps) {
+    HIP_TRY(s, hipGraphLaunch(s->graphN, s->stream));
+    left -= kGraphSteps;
+  }
+  while (left > 0) {
+    HIP_TRY(s, hipGraphLaunch(s->graph1, s->stream));
+    left--;
+  }
+  s->steps += steps;
+  return VLJ_OK;
+}
+
+void decode(const vlj_sim* s, uint32_t r, const long long* slots, const Control& c, uint64_t step,
+            vlj_measurement* m) {
+  const Replica& R = s->rep[r];
+  std::memset(m, 0, sizeof(*m));
+  m->step = step;
+  m->time = (double)step * R.dt;
+  m->kinetic = (double)slots[0] / R.energy_fixed;
+  m->potential = (double)slots[1] / R.energy_fixed;
+  m->virial = (double)slots[2] / R.energy_fixed;
+  for (int k = 0; k < 3; k++) m->momentum[k] = (double)slots[3 + k] / R.momentum_fixed;
+  m->count = R.count;
+  m->volume = R.box[0] * R.box[1] * R.box[2];
+  m->list_rebuilds = c.rebuilds;
+  m->list_overflow = c.overflow;
+  m->thermostat_energy = 0.0;
+  m->condensed = (uint32_t)slots[6];
+  m->pairs_within_cutoff = (uint64_t)slots[7] / 2u;
+  for (int k = 0; k < kSlots; k++) m->raw[k] = slots[k];
+}
+
+int enqueue_measure(vlj_sim* s, int slot) {
+  int rc = prime_if_needed(s);
+  if (rc != VLJ_OK) return rc;
+  Ring& r = s->ring[slot];
+  const size_t R = s->rep.size();
+  HIP_TRY(s, launch_measure(s->B, s->precision, s->shifted, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(r.slots, s->B.slots, R * kSlots * sizeof(long long), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(r.ctl, s->B.ctl, R * sizeof(Control), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(s, hipEventRecord(r.done, s->stream));
+  r.step = s->steps;
+  r.in_flight = true;
+  return VLJ_OK;
+}
+
+int free_slot(const vlj_sim* s) {
+  for (int k = 0; k < kRing; k++)
+    if (!s->ring[k].in_flight) return k;
+  return -1;
+}
+
+int read_dev(vlj_sim* s, void* dst, const void* src, size_t bytes) {
+  HIP_TRY(s, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  return VLJ_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int vlj_create(const vlj_particle* particles, size_t count, const vlj_settings* settings, vlj_sim** out) {
+  if (!settings) return fail(nullptr, VLJ_E_INVALID, "settings must not be NULL");
+  uint64_t offsets[2] = {0, (uint64_t)count};
+  return create(particles, offsets, settings, 1, out);
+}
+
+void vlj_destroy(vlj_sim* sim) {
+  if (!sim) return;
+  free_all(sim);
+  delete sim;
+}
+
+int vlj_step(vlj_sim* sim, uint32_t steps) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  return enqueue_steps(sim, steps);
+}
+
+int vlj_run(vlj_sim* sim, uint64_t steps) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  int rc = enqueue_steps(sim, steps);
+  if (rc != VLJ_OK) return rc;
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  return VLJ_OK;
+}
+
+int vlj_measure(vlj_sim* sim, int32_t* accepted) {
+  if (!sim || !accepted) return fail(sim, VLJ_E_INVALID, "sim and accepted must not be NULL");
+  int slot = free_slot(sim);
+  if (slot < 0) {
+    *accepted = 0;
+    return VLJ_OK;
+  }
+  int rc = enqueue_measure(sim, slot);
+  *accepted = rc == VLJ_OK ? 1 : 0;
+  return rc;
+}
+
+int vlj_poll(vlj_sim* sim, vlj_measurement* out, size_t capacity, size_t* written) {
+  if (!sim || !written || (capacity && !out)) return fail(sim, VLJ_E_INVALID, "bad arguments to vlj_poll");
+  *written = 0;
+  std::vector<std::pair<uint64_t, int>> done;
+  for (int k = 0; k < kRing; k++) {
+    Ring& r = sim->ring[k];
+    if (!r.in_flight) continue;
+    hipError_t e = hipEventQuery(r.done);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) return fail(sim, VLJ_E_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+    done.push_back({r.step, k});
+  }
+  std::sort(done.begin(), done.end());
+  for (auto& d : done) {
+    if (*written >= capacity) break;
+    Ring& r = sim->ring[d.second];
+    decode(sim, 0, r.slots, r.ctl[0], r.step, out + *written);
+    r.in_flight = false;
+    (*written)++;
+  }
+  return VLJ_OK;
+}
+
+int vlj_measure_now(vlj_sim* sim, vlj_measurement* out) {
+  if (!sim || !out) return fail(sim, VLJ_E_INVALID, "sim and out must not be NULL");
+  int slot = free_slot(sim);
+  if (slot < 0) return fail(sim, VLJ_E_STATE, "a measurement is already in flight");
+  int rc = enqueue_measure(sim, slot);
+  if (rc != VLJ_OK) return rc;
+  HIP_TRY(sim, hipEventSynchronize(sim->ring[slot].done));
+  Ring& r = sim->ring[slot];
+  decode(sim, 0, r.slots, r.ctl[0], r.step, out);
+  r.in_flight = false;
+  return VLJ_OK;
+}
+
+static int read_state(vlj_sim* sim, std::vector<double>* pos, std::vector<double>* vel,
+                      std::vector<double>* acc, std::vector<uint32_t>* orig) {
+  const size_t N = sim->n_total;
+  const Buffers& B = sim->B;
+  int rc = VLJ_OK;
+  double* const src[3][3] = {{B.x, B.y, B.z}, {B.vx, B.vy, B.vz}, {B.ax, B.ay, B.az}};
+  std::vector<double>* dst[3] = {pos, vel, acc};
+  std::vector<double> tmp(N);
+  for (int g = 0; g < 3; g++) {
+    if (!dst[g]) continue;
+    dst[g]->assign(3 * N, 0.0);
+    for (int k = 0; k < 3 && rc == VLJ_OK; k++) {
+      rc = read_dev(sim, tmp.data(), src[g][k], N * sizeof(double));
+      for (size_t i = 0; i < N; i++) (*dst[g])[3 * i + k] = tmp[i];
+    }
+  }
+  if (orig && rc == VLJ_OK) {
+    orig->assign(N, 0);
+    rc = read_dev(sim, orig->data(), B.orig, N * 4);
+  }
+  return rc;
+}
+
+int vlj_batch_read_particles(vlj_batch* sim, vlj_particle* out, size_t count) {
+  if (!sim || (count && !out)) return fail(sim, VLJ_E_INVALID, "bad arguments to read_particles");
+  if (count != sim->n_total) return fail(sim, VLJ_E_INVALID, "count must equal the number of particles");
+  std::vector<double> pos, vel;
+  std::vector<uint32_t> orig;
+  int rc = read_state(sim, &pos, &vel, nullptr, &orig);
+  if (rc != VLJ_OK) return rc;
+  for (size_t r = 0; r < sim->rep.size(); r++) {
+    const Replica& R = sim->rep[r];
+    for (uint32_t t = 0; t < R.count; t++) {
+      const size_t slot = R.atom_off + t;
+      vlj_particle& p = out[R.atom_off + orig[slot]];
+      for (int k = 0; k < 3; k++) {
+        p.pos[k] = pos[3 * slot + k];
+        p.vel[k] = vel[3 * slot + k];
+      }
+    }
+  }
+  return VLJ_OK;
+}
+
+int vlj_read_particles(vlj_sim* sim, vlj_particle* out, size_t count) {
+  return vlj_batch_read_particles(sim, out, count);
+}
+
+uint32_t vlj_count(const vlj_sim* sim) { return sim ? sim->rep[0].count : 0u; }
+
+int vlj_box_size(const vlj_sim* sim, double out[3]) {
+  if (!sim || !out) return VLJ_E_INVALID;
+  for (int k = 0; k < 3; k++) out[k] = sim->rep[0].box[k];
+  return VLJ_OK;
+}
+
+double vlj_volume(const vlj_sim* sim) {
+  return sim ? sim->rep[0].box[0] * sim->rep[0].box[1] * sim->rep[0].box[2] : 0.0;
+}
+
+uint64_t vlj_steps(const vlj_sim* sim) { return sim ? sim->steps : 0u; }
+
+double vlj_time(const vlj_sim* sim) { return sim ? (double)sim->steps * sim->rep[0].dt : 0.0; }
+
+int vlj_target_temperature(const vlj_sim* sim, double* temperature, int32_t* has) {
+  if (!sim || !temperature || !has) return VLJ_E_INVALID;
+  const vlj_thermostat& t = sim->host[0].settings.thermostat;
+  *has = t.kind != VLJ_THERMOSTAT_NONE ? 1 : 0;
+  *temperature = *has ? t.temperature : 0.0;
+  return VLJ_OK;
+}
+
+int vlj_set_temperature(vlj_sim* sim, double temperature) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  vlj_thermostat& t = sim->host[0].settings.thermostat;
+  if (t.kind == VLJ_THERMOSTAT_NONE) return fail(sim, VLJ_E_STATE, "set_temperature needs a thermostat");
+  if (!(temperature >= 0.0)) return fail(sim, VLJ_E_INVALID, "temperature must be non-negative");
+  t.temperature = temperature;
+  Replica& R = sim->rep[0];
+  const double c1 = std::exp(-t.friction * R.dt);
+  R.c2 = std::sqrt((1.0 - c1 * c1) * temperature);
+  HIP_TRY(sim, hipMemcpyAsync((void*)&sim->B.rep[0], &R, sizeof(Replica), hipMemcpyHostToDevice, sim->stream));
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  return VLJ_OK;
+}
+
+int vlj_set_condensed_neighbours(vlj_sim* sim, uint32_t neighbours) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  for (size_t r = 0; r < sim->rep.size(); r++) sim->rep[r].near_count = neighbours;
+  HIP_TRY(sim, hipMemcpyAsync((void*)sim->B.rep, sim->rep.data(), sim->rep.size() * sizeof(Replica),
+                              hipMemcpyHostToDevice, sim->stream));
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  return VLJ_OK;
+}
+
+int vlj_sample_profile(vlj_sim* sim) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (sim->host[0].settings.profile_bins == 0)
+    return fail(sim, VLJ_E_STATE, "profiles need Settings::profile_bins above zero");
+  return fail(sim, VLJ_E_UNSUPPORTED, "profiles are not built yet");
+}
+
+int vlj_clear_profile(vlj_sim* sim) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  return VLJ_OK;
+}
+
+int vlj_read_profile(vlj_sim* sim, double* sums, size_t capacity, uint64_t* samples) {
+  (void)sums;
+  (void)capacity;
+  (void)samples;
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (sim->host[0].settings.profile_bins == 0)
+    return fail(sim, VLJ_E_STATE, "profiles need Settings::profile_bins above zero");
+  return fail(sim, VLJ_E_UNSUPPORTED, "profiles are not built yet");
+}
+
+const char* vlj_last_error(const vlj_sim* sim) { return sim ? sim->error.c_str() : g_error.c_str(); }
+
+int vlj_batch_create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings* settings,
+                     size_t replicas, vlj_batch** out) {
+  return create(all, offsets, settings, replicas, out);
+}
+
+int vlj_batch_run(vlj_batch* batch, uint64_t steps) { return vlj_run(batch, steps); }
+
+int vlj_batch_measure_now(vlj_batch* sim, vlj_measurement* per_replica, size_t replicas) {
+  if (!sim || !per_replica) return fail(sim, VLJ_E_INVALID, "bad arguments to batch_measure_now");
+  if (replicas != sim->rep.size()) return fail(sim, VLJ_E_INVALID, "replicas must equal the batch size");
+  int slot = free_slot(sim);
+  if (slot < 0) return fail(sim, VLJ_E_STATE, "a measurement is already in flight");
+  int rc = enqueue_measure(sim, slot);
+  if (rc != VLJ_OK) return rc;
+  Ring& r = sim->ring[slot];
+  HIP_TRY(sim, hipEventSynchronize(r.done));
+  for (size_t k = 0; k < replicas; k++)
+    decode(sim, (uint32_t)k, r.slots + k * kSlots, r.ctl[k], r.step, per_replica + k);
+  r.in_flight = false;
+  return VLJ_OK;
+}
+
+size_t vlj_batch_replicas(const vlj_batch* batch) { return batch ? batch->rep.size() : 0; }
+
+int vlj_fixed_sum(const double* values, size_t count, double fixed_scale, int64_t* out) {
+  if (!out || (count && !values)) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_fixed_sum");
+  vlj_sim tmp;
+  int rc = check_device(&tmp);
+  if (rc != VLJ_OK) return rc;
+  double* d = nullptr;
+  long long* acc = nullptr;
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&d, std::max<size_t>(count, 1) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&acc, kSlots * sizeof(long long));
+  if (e == hipSuccess && count) e = hipMemcpyAsync(d, values, count * sizeof(double), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_fixed_sum(d, count, fixed_scale, acc, st);
+  long long host[kSlots] = {0};
+  if (e == hipSuccess) e = hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (d) (void)hipFree(d);
+  if (acc) (void)hipFree(acc);
+  if (st) (void)hipStreamDestroy(st);
+  if (e != hipSuccess) return fail(nullptr, VLJ_E_HIP, std::string("vlj_fixed_sum: ") + hipGetErrorString(e));
+  *out = host[0];
+  return VLJ_OK;
+}
+
+int vlj_debug_state(vlj_sim* sim, double* pos, double* vel, double* acc, uint32_t* orig) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  std::vector<double> p, v, a;
+  std::vector<uint32_t> o;
+  int rc = read_state(sim, pos ? &p : nullptr, vel ? &v : nullptr, acc ? &a : nullptr, orig ? &o : nullptr);
+  if (rc != VLJ_OK) return rc;
+  if (pos) std::memcpy(pos, p.data(), p.size() * sizeof(double));
+  if (vel) std::memcpy(vel, v.data(), v.size() * sizeof(double));
+  if (acc) std::memcpy(acc, a.data(), a.size() * sizeof(double));
+  if (orig) std::memcpy(orig, o.data(), o.size() * 4);
+  return VLJ_OK;
+}
+
+int vlj_debug_cells(vlj_sim* sim, uint32_t* cell_of, uint32_t* cell_start, uint32_t* sorted) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  const Replica& R = sim->rep[0];
+  const Buffers& B = sim->B;
+  int rc = VLJ_OK;
+  if (cell_of) {
+    rc = read_dev(sim, cell_of, B.cell_of + R.atom_off, (size_t)R.count * 4);
+    for (uint32_t i = 0; i < R.count && rc == VLJ_OK; i++) cell_of[i] -= R.cell_off;
+  }
+  if (cell_start && rc == VLJ_OK) {
+    rc = read_dev(sim, cell_start, B.cell_start + R.cell_off, (size_t)R.total_cells * 4);
+    for (uint32_t c = 0; c < R.total_cells; c++) cell_start[c] -= R.atom_off;
+    cell_start[R.total_cells] = R.count;
+  }
+  if (sorted && rc == VLJ_OK) {
+    rc = read_dev(sim, sorted, B.sorted + R.atom_off, (size_t)R.count * 4);
+    for (uint32_t i = 0; i < R.count && rc == VLJ_OK; i++) sorted[i] -= R.atom_off;
+  }
+  return rc;
+}
+
+int vlj_debug_lists(vlj_sim* sim, uint32_t* counts, uint32_t* entries) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  const Replica& R = sim->rep[0];
+  const Buffers& B = sim->B;
+  std::vector<uint32_t> cnt(R.count);
+  int rc = read_dev(sim, cnt.data(), B.nbr_count + R.atom_off, (size_t)R.count * 4);
+  if (rc != VLJ_OK) return rc;
+  if (counts) std::memcpy(counts, cnt.data(), cnt.size() * 4);
+  if (entries) {
+    std::vector<uint32_t> t((size_t)R.capacity * R.count);
+    rc = read_dev(sim, t.data(), B.nbr + R.list_off, t.size() * 4);
+    if (rc != VLJ_OK) return rc;
+    for (uint32_t i = 0; i < R.count; i++)
+      for (uint32_t k = 0; k < R.capacity; k++)
+        entries[(size_t)i * R.capacity + k] = k < cnt[i] ? t[(size_t)k * R.count + i] - R.atom_off : 0u;
+  }
+  return VLJ_OK;
+}
+
+int vlj_debug_grid(const vlj_sim* sim, uint32_t out[6]) {
+  if (!sim || !out) return VLJ_E_INVALID;
+  const Replica& R = sim->rep[0];
+  out[0] = R.cells[0];
+  out[1] = R.cells[1];
+  out[2] = R.cells[2];
+  out[3] = R.total_cells;
+  out[4] = R.capacity;
+  Control c;
+  if (hipMemcpy(&c, sim->B.ctl, sizeof(Control), hipMemcpyDeviceToHost) != hipSuccess) return VLJ_E_HIP;
+  out[5] = c.rebuilds;
+  return VLJ_OK;
+}
+
+int vlj_debug_params(const vlj_sim* sim, double out[6]) {
+  if (!sim || !out) return VLJ_E_INVALID;
+  out[0] = sim->host[0].energy_scale;
+  out[1] = sim->host[0].momentum_scale;
+  out[2] = sim->rep[0].shift;
+  out[3] = sim->rep[0].force_shift;
+  out[4] = sim->rep[0].c1;
+  out[5] = sim->rep[0].c2;
+  return VLJ_OK;
+}
+
+int vlj_timed_run(vlj_sim* sim, uint64_t steps, double* force_ms, double* total_ms) {
+  if (!sim || !force_ms || !total_ms) return fail(sim, VLJ_E_INVALID, "bad arguments to vlj_timed_run");
+  int rc = prime_if_needed(sim);
+  if (rc != VLJ_OK) return rc;
+  std::vector<hipEvent_t> ev(2 * steps + 2, nullptr);
+  for (auto& e : ev) HIP_TRY(sim, hipEventCreate(&e));
+  HIP_TRY(sim, hipEventRecord(ev[2 * steps], sim->stream));
+  for (uint64_t t = 0; t < steps && rc == VLJ_OK; t++) {
+    HIP_TRY(sim, launch_drift(sim->B, sim->any_langevin, sim->stream));
+    HIP_TRY(sim, launch_rebuild(sim->B, sim->stream));
+    HIP_TRY(sim, hipEventRecord(ev[2 * t], sim->stream));
+    HIP_TRY(sim, launch_force(sim->B, sim->precision, sim->shifted, true, sim->stream));
+    HIP_TRY(sim, hipEventRecord(ev[2 * t + 1], sim->stream));
+  }
+  HIP_TRY(sim, hipEventRecord(ev[2 * steps + 1], sim->stream));
+  HIP_TRY(sim, hipEventSynchronize(ev[2 * steps + 1]));
+  double f = 0.0;
+  for (uint64_t t = 0; t < steps; t++) {
+    float ms = 0.0f;
+    HIP_TRY(sim, hipEventElapsedTime(&ms, ev[2 * t], ev[2 * t + 1]));
+    f += ms;
+  }
+  float total = 0.0f;
+  HIP_TRY(sim, hipEventElapsedTime(&total, ev[2 * steps], ev[2 * steps + 1]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  sim->steps += steps;
+  *force_ms = f;
+  *total_ms = total;
+  return VLJ_OK;
+}
+
+// ---- lattices (lennard_jones.rs:1377-1441), host only ----
+
+static uint64_t xorshift(uint64_t* s) {
+  *s ^= *s << 13;
+  *s ^= *s >> 7;
+  *s ^= *s << 17;
+  return *s;
+}
+
+static double lattice_random(uint64_t* s) {
+  xorshift(s);
+  return ((double)(*s >> 11) + 0.5) / 9007199254740992.0;
+}
+
+static double lattice_gaussian(uint64_t* s) {
+  double r = std::sqrt(-2.0 * std::log(lattice_random(s)));
+  double u = lattice_random(s);
+  return r * std::cos(6.283185307179586 * u);
+}
+
+static void zero_momentum(size_t count, vlj_particle* out) {
+  for (int k = 0; k < 3; k++) {
+    double sum = 0.0;
+    for (size_t i = 0; i < count; i++) sum += out[i].vel[k];
+    double mean = sum / (double)count;
+    for (size_t i = 0; i < count; i++) out[i].vel[k] -= mean;
+  }
+}
+
+static void lattice_sites(size_t count, const double size[3], const size_t sites[3], double temperature,
+                          uint64_t seed, vlj_particle* out) {
+  double gap[3];
+  for (int k = 0; k < 3; k++) gap[k] = size[k] / (double)sites[k];
+  uint64_t s = seed | 1u;
+  double st = std::sqrt(temperature);
+  for (size_t i = 0; i < count; i++) {
+    size_t c[3] = {i % sites[0], (i / sites[0]) % sites[1], i / (sites[0] * sites[1])};
+    for (int k = 0; k < 3; k++) out[i].pos[k] = ((double)c[k] + 0.5) * gap[k];
+    for (int k = 0; k < 3; k++) out[i].vel[k] = lattice_gaussian(&s) * st;
+  }
+  if (count) zero_momentum(count, out);
+}
+
+int vlj_lattice(size_t count, double box_size, double temperature, uint64_t seed, vlj_particle* out) {
+  if (count && !out) return fail(nullptr, VLJ_E_INVALID, "out must not be NULL");
+  size_t per_side = (size_t)std::ceil(std::cbrt((double)count));
+  double size[3] = {box_size, box_size, box_size};
+  size_t sites[3] = {per_side, per_side, per_side};
+  lattice_sites(count, size, sites, temperature, seed, out);
+  return VLJ_OK;
+}
+
+int vlj_lattice_box(size_t count, const double size[3], double temperature, uint64_t seed, vlj_particle* out) {
+  if ((count && !out) || !size) return fail(nullptr, VLJ_E_INVALID, "bad arguments to lattice_box");
+  if (count == 0) return VLJ_OK;
+  double spacing = std::cbrt(size[0] * size[1] * size[2] / (double)count);
+  size_t sites[3];
+  for (int k = 0; k < 3; k++) {
+    double r = std::round(size[k] / spacing);
+    sites[k] = r < 1.0 ? 1 : (size_t)r;
+  }
+  while (sites[0] * sites[1] * sites[2] < count) {
+    int widest = 0;  // Rust's max_by keeps the last of equal maxima
+    for (int k = 1; k < 3; k++)
+      if (size[k] / (double)sites[k] >= size[widest] / (double)sites[widest]) widest = k;
+    sites[widest] += 1;
+  }
+  lattice_sites(count, size, sites, temperature, seed, out);
+  return VLJ_OK;
+}
+
+int vlj_fcc(uint32_t n, double density, double temperature, uint64_t seed, vlj_particle* out, double* side) {
+  if (!out || !side || n == 0 || !(density > 0.0)) return fail(nullptr, VLJ_E_INVALID, "bad arguments to fcc");
+  const double a0 = std::cbrt(4.0 / density);
+  const double L = (double)(float)(a0 * (double)n);
+  const double a = L / (double)n;
+  static const double basis[4][3] = {{0, 0, 0}, {0.5, 0.5, 0}, {0.5, 0, 0.5}, {0, 0.5, 0.5}};
+  uint64_t s = seed | 1u;
+  const double st = std::sqrt(temperature);
+  size_t i = 0;
+  for (uint32_t cz = 0; cz < n; cz++)
+    for (uint32_t cy = 0; cy < n; cy++)
+      for (uint32_t cx = 0; cx < n; cx++)
+        for (int b = 0; b < 4; b++, i++) {
+          const double c[3] = {(double)cx, (double)cy, (double)cz};
+          for (int k = 0; k < 3; k++) out[i].pos[k] = (c[k] + basis[b][k] + 0.25) * a;
+          for (int k = 0; k < 3; k++) out[i].vel[k] = lattice_gaussian(&s) * st;
+        }
+  zero_momentum(i, out);
+  *side = L;
+  return VLJ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,418 @@
+"""Lennard-Jones particles in a periodic rectangular box, on an MI355X.
+
+Python mirror of `virial_particles::lennard_jones`
+(crates/virial-particles/src/lennard_jones.rs): the same types, builder,
+step and measurement API and the same panic messages (raised here as
+exceptions), over the C ABI of libvirial_lj.so (include/virial_lj.h). All
+computation happens in the HIP library; this module only marshals.
+
+Reduced units (sigma = epsilon = mass = k_B = 1). Pressure from the virial
+theorem, P = (2 KE + W) / (3 V) (lennard_jones.rs:110-131).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Union
+
+import numpy as np
+
+from . import _lib as L
+
+PARTICLE_DTYPE = np.dtype([("pos", "<f8", (3,)), ("vel", "<f8", (3,))])
+FP64 = 0
+FP32 = 1
+
+
+class LennardJonesError(RuntimeError):
+    """A failed library call. `code` is the VLJ_E_* status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+def _check(code: int, handle=None) -> None:
+    if code != L.OK:
+        raise LennardJonesError(code, L.lib().vlj_last_error(handle).decode())
+
+
+@dataclass
+class Particle:
+    """A particle (mass 1) (lennard_jones.rs:25-30)."""
+
+    pos: Sequence[float]
+    vel: Sequence[float]
+
+
+@dataclass
+class Langevin:
+    """Langevin thermostat, BAOAB splitting (lennard_jones.rs:32-40)."""
+
+    temperature: float
+    friction: float
+    seed: int
+
+
+@dataclass
+class NoseHoover:
+    """Nose-Hoover thermostat (lennard_jones.rs:42-52)."""
+
+    temperature: float
+    period: float
+
+
+Thermostat = Optional[Union[Langevin, NoseHoover]]
+
+
+@dataclass
+class Settings:
+    """Box, cutoff, time step and thermostat (lennard_jones.rs:63-82), plus
+    the pair-force precision (FP64, or FP32 on FP64 displacements)."""
+
+    box_size: Sequence[float]
+    cutoff: float
+    dt: float
+    thermostat: Thermostat = None
+    shifted_force: bool = False
+    skin: float = 0.3
+    profile_bins: int = 0
+    precision: int = FP64
+
+    def to_c(self) -> L.Settings:
+        s = L.Settings()
+        for k in range(3):
+            s.box_size[k] = float(self.box_size[k])
+        s.cutoff = self.cutoff
+        s.dt = self.dt
+        s.skin = self.skin
+        t = self.thermostat
+        if t is None:
+            s.thermostat.kind = 0
+        elif isinstance(t, Langevin):
+            s.thermostat.kind = 1
+            s.thermostat.temperature = t.temperature
+            s.thermostat.friction = t.friction
+            s.thermostat.seed = int(t.seed)
+        elif isinstance(t, NoseHoover):
+            s.thermostat.kind = 2
+            s.thermostat.temperature = t.temperature
+            s.thermostat.period = t.period
+        else:
+            raise TypeError("thermostat must be None, Langevin or NoseHoover")
+        s.shifted_force = 1 if self.shifted_force else 0
+        s.profile_bins = int(self.profile_bins)
+        s.precision = int(self.precision)
+        return s
+
+
+@dataclass
+class Measurement:
+    """Totals at one step (lennard_jones.rs:84-131)."""
+
+    step: int
+    time: float
+    kinetic: float
+    potential: float
+    virial: float
+    momentum: List[float]
+    count: int
+    volume: float
+    list_rebuilds: int
+    list_overflow: int
+    thermostat_energy: float
+    condensed: int
+    pairs_within_cutoff: int = 0
+    raw: List[int] = field(default_factory=list)
+
+    @classmethod
+    def from_c(cls, m: L.Measurement) -> "Measurement":
+        return cls(m.step, m.time, m.kinetic, m.potential, m.virial, list(m.momentum), m.count, m.volume,
+                   m.list_rebuilds, m.list_overflow, m.thermostat_energy, m.condensed, m.pairs_within_cutoff,
+                   list(m.raw))
+
+    def energy(self) -> float:
+        return self.kinetic + self.potential
+
+    def temperature(self, dof: float) -> float:
+        return 2.0 * self.kinetic / dof
+
+    def pressure(self) -> float:
+        return (2.0 * self.kinetic + self.virial) / (3.0 * self.volume)
+
+    def extended_energy(self) -> float:
+        return self.energy() + self.thermostat_energy
+
+
+def _as_particle_array(particles) -> np.ndarray:
+    """Particles as a C-contiguous array of vlj_particle (48 bytes each)."""
+    if isinstance(particles, np.ndarray) and particles.dtype == PARTICLE_DTYPE:
+        return np.ascontiguousarray(particles)
+    if isinstance(particles, tuple) and len(particles) == 2:
+        pos, vel = (np.asarray(a, dtype=np.float64).reshape(-1, 3) for a in particles)
+        out = np.zeros(pos.shape[0], dtype=PARTICLE_DTYPE)
+        out["pos"] = pos
+        out["vel"] = vel
+        return out
+    out = np.zeros(len(particles), dtype=PARTICLE_DTYPE)
+    for i, p in enumerate(particles):
+        out["pos"][i] = p.pos
+        out["vel"][i] = p.vel
+    return out
+
+
+def particles_from_arrays(pos, vel) -> np.ndarray:
+    return _as_particle_array((pos, vel))
+
+
+def _pptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(L.Particle))
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _uptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def to_particles(a: np.ndarray) -> List[Particle]:
+    return [Particle(list(map(float, r["pos"])), list(map(float, r["vel"]))) for r in a]
+
+
+class LennardJones:
+    """Lennard-Jones particles on the GPU (lennard_jones.rs:332-1375).
+
+    `particles` is a list of Particle, a (pos, vel) pair of (n, 3) arrays or
+    a PARTICLE_DTYPE array. The reference's constructor also takes the wgpu
+    device and queue; here the current HIP device is used.
+    """
+
+    def __init__(self, particles, settings: Settings):
+        arr = _as_particle_array(particles)
+        self._settings = settings
+        h = C.c_void_p()
+        cs = settings.to_c()
+        _check(L.lib().vlj_create(_pptr(arr), arr.shape[0], C.byref(cs), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().vlj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, code: int) -> None:
+        _check(code, self._h)
+
+    # ---- stepping (lennard_jones.rs:1092-1138, 1274-1284)
+    def step(self, steps: int) -> None:
+        self._c(L.lib().vlj_step(self._h, steps))
+
+    def run(self, steps: int) -> None:
+        self._c(L.lib().vlj_run(self._h, steps))
+
+    # ---- measurements (lennard_jones.rs:1141-1260)
+    def measure(self) -> bool:
+        ok = C.c_int32()
+        self._c(L.lib().vlj_measure(self._h, C.byref(ok)))
+        return bool(ok.value)
+
+    def after_submit(self) -> None:
+        """Kept for call-site shape: the library's read-back needs no mapping step."""
+
+    def poll(self) -> List[Measurement]:
+        out = (L.Measurement * 3)()
+        n = C.c_size_t()
+        self._c(L.lib().vlj_poll(self._h, out, 3, C.byref(n)))
+        return [Measurement.from_c(out[k]) for k in range(n.value)]
+
+    def measure_now(self) -> Measurement:
+        m = L.Measurement()
+        self._c(L.lib().vlj_measure_now(self._h, C.byref(m)))
+        return Measurement.from_c(m)
+
+    # ---- state (lennard_jones.rs:1354-1374)
+    def read_particle_array(self) -> np.ndarray:
+        out = np.zeros(self.count(), dtype=PARTICLE_DTYPE)
+        self._c(L.lib().vlj_read_particles(self._h, _pptr(out), out.shape[0]))
+        return out
+
+    def read_particles(self) -> List[Particle]:
+        return to_particles(self.read_particle_array())
+
+    # ---- accessors (lennard_jones.rs:870-1030)
+    def count(self) -> int:
+        return L.lib().vlj_count(self._h)
+
+    def box_size(self) -> List[float]:
+        out = (C.c_double * 3)()
+        self._c(L.lib().vlj_box_size(self._h, out))
+        return list(out)
+
+    def volume(self) -> float:
+        return L.lib().vlj_volume(self._h)
+
+    def steps(self) -> int:
+        return L.lib().vlj_steps(self._h)
+
+    def time(self) -> float:
+        return L.lib().vlj_time(self._h)
+
+    def target_temperature(self) -> Optional[float]:
+        t = C.c_double()
+        has = C.c_int32()
+        self._c(L.lib().vlj_target_temperature(self._h, C.byref(t), C.byref(has)))
+        return t.value if has.value else None
+
+    def set_temperature(self, temperature: float) -> None:
+        self._c(L.lib().vlj_set_temperature(self._h, temperature))
+
+    def set_condensed_neighbours(self, neighbours: int) -> None:
+        self._c(L.lib().vlj_set_condensed_neighbours(self._h, neighbours))
+
+    # ---- profiles (lennard_jones.rs:1167-1191, 1297-1325)
+    def sample_profile(self) -> None:
+        self._c(L.lib().vlj_sample_profile(self._h))
+
+    def sample_profile_now(self) -> None:
+        self.sample_profile()
+
+    def clear_profile(self) -> None:
+        self._c(L.lib().vlj_clear_profile(self._h))
+
+    # ---- inspection (parity tests; no counterpart in the reference)
+    def debug_state(self):
+        n = self.count()
+        pos, vel, acc = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        orig = np.zeros(n, dtype=np.uint32)
+        self._c(L.lib().vlj_debug_state(self._h, _dptr(pos), _dptr(vel), _dptr(acc), _uptr(orig)))
+        return pos, vel, acc, orig
+
+    def debug_grid(self) -> dict:
+        out = np.zeros(6, dtype=np.uint32)
+        self._c(L.lib().vlj_debug_grid(self._h, _uptr(out)))
+        return {"cells": [int(x) for x in out[:3]], "total_cells": int(out[3]), "capacity": int(out[4]),
+                "rebuilds": int(out[5])}
+
+    def debug_params(self) -> dict:
+        out = np.zeros(6)
+        self._c(L.lib().vlj_debug_params(self._h, _dptr(out)))
+        keys = ["energy_scale", "momentum_scale", "shift", "force_shift", "c1", "c2"]
+        return dict(zip(keys, map(float, out)))
+
+    def debug_cells(self):
+        g = self.debug_grid()
+        n = self.count()
+        cell_of = np.zeros(n, dtype=np.uint32)
+        start = np.zeros(g["total_cells"] + 1, dtype=np.uint32)
+        sorted_ = np.zeros(n, dtype=np.uint32)
+        self._c(L.lib().vlj_debug_cells(self._h, _uptr(cell_of), _uptr(start), _uptr(sorted_)))
+        return cell_of, start, sorted_
+
+    def debug_lists(self):
+        g = self.debug_grid()
+        n = self.count()
+        counts = np.zeros(n, dtype=np.uint32)
+        entries = np.zeros((n, g["capacity"]), dtype=np.uint32)
+        self._c(L.lib().vlj_debug_lists(self._h, _uptr(counts), _uptr(entries)))
+        return counts, entries
+
+    def timed_run(self, steps: int):
+        """(force-kernel ms summed over the steps, total ms) from HIP events."""
+        f, t = C.c_double(), C.c_double()
+        self._c(L.lib().vlj_timed_run(self._h, steps, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+
+class Batch:
+    """Many independent replicas in the same launches (no counterpart in the
+    reference). `replicas` is a list of (particles, Settings)."""
+
+    def __init__(self, replicas):
+        arrays = [_as_particle_array(p) for p, _ in replicas]
+        self.sizes = [a.shape[0] for a in arrays]
+        allp = np.concatenate(arrays) if arrays else np.zeros(0, dtype=PARTICLE_DTYPE)
+        offsets = np.zeros(len(arrays) + 1, dtype=np.uint64)
+        offsets[1:] = np.cumsum(self.sizes)
+        settings = (L.Settings * len(replicas))(*[s.to_c() for _, s in replicas])
+        h = C.c_void_p()
+        _check(L.lib().vlj_batch_create(_pptr(allp), offsets.ctypes.data_as(C.POINTER(C.c_uint64)), settings,
+                                        len(replicas), C.byref(h)))
+        self._h = h
+        self.replicas = len(replicas)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().vlj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, steps: int) -> None:
+        _check(L.lib().vlj_batch_run(self._h, steps), self._h)
+
+    def measure_now(self) -> List[Measurement]:
+        out = (L.Measurement * self.replicas)()
+        _check(L.lib().vlj_batch_measure_now(self._h, out, self.replicas), self._h)
+        return [Measurement.from_c(m) for m in out]
+
+    def read_particle_array(self) -> np.ndarray:
+        out = np.zeros(sum(self.sizes), dtype=PARTICLE_DTYPE)
+        _check(L.lib().vlj_batch_read_particles(self._h, _pptr(out), out.shape[0]), self._h)
+        return out
+
+
+def lattice(count: int, box_size: float, temperature: float, seed: int) -> List[Particle]:
+    """Simple cubic lattice filling the cube (lennard_jones.rs:1379-1382)."""
+    return to_particles(lattice_array(count, box_size, temperature, seed))
+
+
+def lattice_array(count: int, box_size: float, temperature: float, seed: int) -> np.ndarray:
+    out = np.zeros(count, dtype=PARTICLE_DTYPE)
+    _check(L.lib().vlj_lattice(count, box_size, temperature, seed, _pptr(out)))
+    return out
+
+
+def lattice_box(count: int, size, temperature: float, seed: int) -> List[Particle]:
+    """Lattice filling a rectangular box (lennard_jones.rs:1388-1399)."""
+    return to_particles(lattice_box_array(count, size, temperature, seed))
+
+
+def lattice_box_array(count: int, size, temperature: float, seed: int) -> np.ndarray:
+    out = np.zeros(count, dtype=PARTICLE_DTYPE)
+    s = (C.c_double * 3)(*map(float, size))
+    _check(L.lib().vlj_lattice_box(count, s, temperature, seed, _pptr(out)))
+    return out
+
+
+def fcc_array(cells_per_side: int, density: float, temperature: float, seed: int):
+    """FCC lattice of 4 n^3 particles; returns (particles, side)."""
+    out = np.zeros(4 * cells_per_side ** 3, dtype=PARTICLE_DTYPE)
+    side = C.c_double()
+    _check(L.lib().vlj_fcc(cells_per_side, density, temperature, seed, _pptr(out), C.byref(side)))
+    return out, side.value
+
+
+def fixed_sum(values, fixed_scale: float) -> int:
+    """Device 64-bit fixed-point sum of round-ties-even(values * fixed_scale)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    out = C.c_int64()
+    _check(L.lib().vlj_fixed_sum(_dptr(v), v.shape[0], fixed_scale, C.byref(out)))
+    return out.value
