@@ -198,6 +198,11 @@ int vlj_debug_grid(const vlj_sim *sim, uint32_t out[6]);
  * force_shift, c1, c2. */
 int vlj_debug_params(const vlj_sim *sim, double out[6]);
 
+/* Table entry that workgroup `block` of a `blocks`-workgroup launch works
+ * on: workgroups go round-robin to the 8 XCDs, and each XCD gets one
+ * contiguous share of the table. A bijection on [0, blocks). */
+uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks);
+
 /* Kernel timing with HIP events on the handle's stream: runs `steps` steps
  * launching the kernels directly (not from the captured graph) with an event
  * pair around every pair-force launch; returns the summed force-kernel time

@@ -7,10 +7,11 @@
  * same way) can follow the identical sequence of IEEE operations.
  *
  * Arithmetic rules shared with the HIP kernels:
- *  R1 nearest image, per axis: d = xj - xi; d > L/2 -> d - L; d < -L/2 -> d + L.
- *     (lennard_jones.wgsl:198-202 uses d - L round(d/L); both give the same
- *     d for every |d| not within rounding of L/2, i.e. for every pair that
- *     can be listed, since L >= 3 (rc + skin).)
+ *  R1 positions are wrapped into the box (x >= L -> x - L, x < 0 -> x + L)
+ *     when the lists are rebuilt, not every step: between rebuilds they move
+ *     continuously, so a pair's periodic image fixed at the rebuild stays
+ *     right until the next one (the reference wraps every step,
+ *     lennard_jones.wgsl:188-196; positions read back are wrapped).
  *     r2 = fma(dz, dz, fma(dy, dy, dx * dx)).
  *  R2 cell of a position: c = clamp(floor((float)x / cs_f32), 0, n - 1) in
  *     f32 per axis, index (cz ny + cy) nx + cx (grid.wgsl:11-20; the
@@ -18,26 +19,33 @@
  *  R3 cell order: slots ascending within each cell (grid.wgsl:129-148).
  *  R4 list: for dz, dy, dx in -1..1 (dz outermost) the cell (c + d + n) % n,
  *     slots in cell order, j != i with r2 < (rc + skin)^2 in FP64
- *     (lennard_jones.wgsl:454-499).
- *  R5 pair, FP64: inv2 = 1 / r2; s6 = inv2 inv2 inv2; rf = s6 fma(48, s6, -24);
- *     u = fma(4 s6, s6, -4 s6) - shift; shifted force: r = sqrt(r2),
- *     rf = fma(-r, fs, rf), u = fma(r - rc, fs, u); fscal = rf inv2;
- *     a -= d fscal as fma(-d, fscal, a) in list order (lennard_jones.wgsl:501-545).
- *  R5f pair, FP32 forces: cutoff test and d as R1/R5 in FP64, then in f32:
+ *     (lennard_jones.wgsl:454-499). The displacement is taken to the image
+ *     of j's cell seen from i's cell, d = (xj + s L) - xi with s = -1 where
+ *     c + d wraps below 0, +1 where it wraps past n - 1, else 0; for n >= 3
+ *     this is the nearest image of every pair within one cell width
+ *     (the reference's nearest(), lennard_jones.wgsl:198-202). Each entry
+ *     keeps its image s.
+ *  R5 pair, FP64, over the list in order with d = (xj + s L) - xi and the
+ *     entry's image s: r2 < rc^2; inv2 = 1 / r2; s6 = inv2 inv2 inv2;
+ *     rf = s6 fma(48, s6, -24); u = fma(4 s6, s6, -4 s6) - shift; shifted
+ *     force: r = sqrt(r2), rf = fma(-r, fs, rf), u = fma(r - rc, fs, u);
+ *     fscal = rf inv2; a -= d fscal as fma(-d, fscal, a) (lennard_jones.wgsl:501-545).
+ *  R5f pair, FP32 forces: cutoff test and d as R5 in FP64, then in f32:
  *     q = (float)r2, inv2 = 1/q, s6, rf, fscal as R5; (float)d * fscal
  *     accumulated in FP64.
- *  R6 drift (velocity Verlet): v = fma(a, dt/2, v); x = fma(v, dt, x); wrap
- *     x >= L -> x - L, x < 0 -> x + L; moved^2 = r2 of nearest(x - x_build)
- *     (lennard_jones.wgsl:204-276).
+ *  R6 drift (velocity Verlet): v = fma(a, dt/2, v); x = fma(v, dt, x);
+ *     moved^2 = r2 of x - x_build (lennard_jones.wgsl:204-276).
  *  R7 kick: v = fma(a, dt/2, v) (lennard_jones.wgsl:555-574).
- *  R8 rebuild iff never built or max moved^2 > (skin/2)^2; reorder into cell
- *     order on every 4th rebuild counting from the first (lennard_jones.wgsl:278-307).
+ *  R8 rebuild iff never built or max moved^2 > (skin/2)^2, and then move the
+ *     atoms into cell order (lennard_jones.wgsl:278-307 reorders on every 4th
+ *     rebuild only; here every rebuild does, so each cell owns one contiguous
+ *     range of slots).
  *  R9 measurement: per atom ke = 0.5 fma(vz, vz, fma(vy, vy, vx vx)), pe and
  *     w summed in list order as pe += 0.5 u, w += 0.5 rf; each per-atom value
  *     goes to 64-bit fixed point as llrint(value * scale * 2^32) and the
  *     integers are summed (lennard_jones.wgsl:576-626, fixed.wgsl:1-15).
  *  R12 Langevin BAOAB: v = fma(a, dt/2, v); x = fma(v, dt/2, x);
- *     v = fma(c2, n, c1 v); x = fma(v, dt/2, x); wrap (lennard_jones.wgsl:249-266),
+ *     v = fma(c2, n, c1 v); x = fma(v, dt/2, x) (lennard_jones.wgsl:249-266),
  *     with normals from Philox4x32-10 and Box-Muller (vo_langevin_noise).
  */
 #include "oracle_lj.h"
@@ -56,6 +64,7 @@ struct vo_sim {
     double *x[3], *v[3], *a[3], *xb[3];
     uint32_t *orig, *cell_of, *sorted, *cell_start, *cell_count;
     uint32_t *nbr_count, *nbr;
+    uint8_t *nbr_image; /* (sx + 1) + 3 (sy + 1) + 9 (sz + 1) per entry */
     double *pe, *w;
     uint32_t *near, *within;
     uint32_t rebuilds, overflow;
@@ -599,6 +608,7 @@ vo_sim *vo_create(const double *pos, const double *vel, size_t n, const double b
     s->cell_count = (uint32_t *)xcalloc(s->p.total_cells, 4);
     s->nbr_count = (uint32_t *)xcalloc(n, 4);
     s->nbr = (uint32_t *)xcalloc(n * (size_t)s->p.capacity, 4);
+    s->nbr_image = (uint8_t *)xcalloc(n * (size_t)s->p.capacity, 1);
     s->pe = (double *)xcalloc(n, sizeof(double));
     s->w = (double *)xcalloc(n, sizeof(double));
     s->near = (uint32_t *)xcalloc(n, 4);
@@ -630,6 +640,7 @@ void vo_destroy(vo_sim *s)
     free(s->cell_count);
     free(s->nbr_count);
     free(s->nbr);
+    free(s->nbr_image);
     free(s->pe);
     free(s->w);
     free(s->near);
@@ -641,14 +652,14 @@ void vo_get_params(const vo_sim *s, vo_params *out) { *out = s->p; }
 void vo_set_near_count(vo_sim *s, uint32_t near_count) { s->p.near_count = near_count; }
 uint32_t vo_rebuilds(const vo_sim *s) { return s->rebuilds; }
 
-/* R1 */
-static inline double nearest(double d, double side, double half)
+/* R1: one box length, for positions at most a skin beyond the box */
+static inline double wrap_step(double x, double side)
 {
-    if (d > half)
-        return d - side;
-    if (d < -half)
-        return d + side;
-    return d;
+    if (x >= side)
+        return x - side;
+    if (x < 0.0)
+        return x + side;
+    return x;
 }
 
 /* grid.wgsl:22-148: bin, prefix sum, then cell order ascending by slot
@@ -724,17 +735,25 @@ static void build_lists(vo_sim *s)
                     int32_t ncx = (cx + dx + nx) % nx, ncy = (cy + dy + ny) % ny,
                             ncz = (cz + dz + nz) % nz;
                     uint32_t cell = (uint32_t)((ncz * ny + ncy) * nx + ncx);
+                    /* periodic image of the neighbour cell: -L, 0 or +L per axis */
+                    int ix = cx + dx < 0 ? -1 : (cx + dx >= nx ? 1 : 0);
+                    int iy = cy + dy < 0 ? -1 : (cy + dy >= ny ? 1 : 0);
+                    int iz = cz + dz < 0 ? -1 : (cz + dz >= nz ? 1 : 0);
+                    double sx = ix * p->box[0], sy = iy * p->box[1], sz = iz * p->box[2];
+                    uint8_t code = (uint8_t)((ix + 1) + 3 * (iy + 1) + 9 * (iz + 1));
                     for (uint32_t t = s->cell_start[cell]; t < s->cell_start[cell + 1]; t++) {
                         uint32_t j = s->sorted[t];
                         if ((int32_t)j == i)
                             continue;
-                        double ddx = nearest(s->x[0][j] - xi, p->box[0], p->half_box[0]);
-                        double ddy = nearest(s->x[1][j] - yi, p->box[1], p->half_box[1]);
-                        double ddz = nearest(s->x[2][j] - zi, p->box[2], p->half_box[2]);
+                        double ddx = (s->x[0][j] + sx) - xi;
+                        double ddy = (s->x[1][j] + sy) - yi;
+                        double ddz = (s->x[2][j] + sz) - zi;
                         double r2 = fma(ddz, ddz, fma(ddy, ddy, ddx * ddx));
                         if (r2 < p->list_r2) {
-                            if (found < cap)
+                            if (found < cap) {
                                 row[found] = j;
+                                s->nbr_image[(size_t)i * cap + found] = code;
+                            }
                             found++;
                         }
                     }
@@ -757,12 +776,13 @@ static void update_lists(vo_sim *s)
     s->max_moved2 = 0.0;
     if (!rebuild)
         return;
-    int do_reorder = (s->rebuilds % 4u) == 0u;
     s->rebuilds++;
     s->built = 1;
+    for (int k = 0; k < 3; k++)
+        for (uint32_t i = 0; i < s->p.count; i++)
+            s->x[k][i] = wrap_step(s->x[k][i], s->p.box[k]);
     bin_and_sort(s);
-    if (do_reorder)
-        reorder(s);
+    reorder(s);
     build_lists(s);
 }
 
@@ -775,11 +795,16 @@ static void interactions(vo_sim *s, int32_t i, double acc[3], int energies)
     uint32_t near = 0, within = 0;
     uint32_t cnt = s->nbr_count[i];
     const uint32_t *row = s->nbr + (size_t)i * p->capacity;
+    const uint8_t *img = s->nbr_image + (size_t)i * p->capacity;
     for (uint32_t k = 0; k < cnt; k++) {
         uint32_t j = row[k];
-        double dx = nearest(s->x[0][j] - xi, p->box[0], p->half_box[0]);
-        double dy = nearest(s->x[1][j] - yi, p->box[1], p->half_box[1]);
-        double dz = nearest(s->x[2][j] - zi, p->box[2], p->half_box[2]);
+        int code = img[k];
+        double sx = (code % 3 - 1) * p->box[0];
+        double sy = ((code / 3) % 3 - 1) * p->box[1];
+        double sz = (code / 9 - 1) * p->box[2];
+        double dx = (s->x[0][j] + sx) - xi;
+        double dy = (s->x[1][j] + sy) - yi;
+        double dz = (s->x[2][j] + sz) - zi;
         double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (!(r2 < p->rc2))
             continue;
@@ -863,15 +888,6 @@ void vo_prime(vo_sim *s)
     s->primed = 1;
 }
 
-static inline double wrap_step(double x, double side)
-{
-    if (x >= side)
-        return x - side;
-    if (x < 0.0)
-        return x + side;
-    return x;
-}
-
 /* R6 / R12, lennard_jones.wgsl:204-276 */
 static void drift(vo_sim *s)
 {
@@ -895,14 +911,13 @@ static void drift(vo_sim *s)
             } else {
                 xk = fma(v, p->dt, xk);
             }
-            xk = wrap_step(xk, p->box[k]);
             s->v[k][i] = v;
             s->x[k][i] = xk;
             x[k] = xk;
         }
-        double dx = nearest(x[0] - s->xb[0][i], p->box[0], p->half_box[0]);
-        double dy = nearest(x[1] - s->xb[1][i], p->box[1], p->half_box[1]);
-        double dz = nearest(x[2] - s->xb[2][i], p->box[2], p->half_box[2]);
+        double dx = x[0] - s->xb[0][i];
+        double dy = x[1] - s->xb[1][i];
+        double dz = x[2] - s->xb[2][i];
         double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (d2 > maxd)
             maxd = d2;
@@ -984,14 +999,14 @@ void vo_read_state(const vo_sim *s, double *pos, double *vel, double *acc, uint3
         memcpy(orig, s->orig, (size_t)n * 4);
 }
 
-/* lennard_jones.rs:1354-1374: original order */
+/* lennard_jones.rs:1354-1374: original order, positions wrapped into the box */
 void vo_read_particles(const vo_sim *s, double *pos, double *vel)
 {
     uint32_t n = s->p.count;
     for (uint32_t t = 0; t < n; t++) {
         uint32_t i = s->orig[t];
         for (int k = 0; k < 3; k++) {
-            pos[3 * i + k] = s->x[k][t];
+            pos[3 * i + k] = wrap_step(s->x[k][t], s->p.box[k]);
             vel[3 * i + k] = s->v[k][t];
         }
     }

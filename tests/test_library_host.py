@@ -1,0 +1,109 @@
+"""Host-side checks of libvirial_lj.so that need no GPU: it loads, exports
+every symbol include/virial_lj.h declares, validates arguments with the
+reference's messages before touching a device, and its host lattices match
+the oracle bit for bit."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "virial_lj.h")
+
+
+def header_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(vlj_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from virial_particles import _lib
+
+    L = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    for name in syms:
+        assert hasattr(L, name), name
+    assert set(syms) == set(_lib.SIGNATURES)
+
+
+def test_create_validates_before_touching_a_device(lj):
+    pos = np.zeros((10, 3))
+    with pytest.raises(lj.LennardJonesError, match="at least three \\(cutoff \\+ skin\\) wide") as e:
+        lj.LennardJones((pos, pos), lj.Settings([8.0, 20.0, 20.0], 2.5, 0.005))
+    assert e.value.code == 1
+    with pytest.raises(lj.LennardJonesError, match="Nose-Hoover") as e:
+        lj.LennardJones((pos, pos), lj.Settings([20.0] * 3, 2.5, 0.005, thermostat=lj.NoseHoover(1.0, 0.5)))
+    assert e.value.code == 4
+    with pytest.raises(lj.LennardJonesError, match="finite"):
+        bad = pos.copy()
+        bad[0, 0] = np.nan
+        lj.LennardJones((bad, pos), lj.Settings([20.0] * 3, 2.5, 0.005))
+
+
+def test_batch_replicas_share_precision(lj):
+    pos = np.random.default_rng(1).uniform(0, 20, (10, 3))
+    a = (pos, np.zeros_like(pos))
+    with pytest.raises(lj.LennardJonesError, match="share"):
+        lj.Batch([(a, lj.Settings([20.0] * 3, 2.5, 0.005)), (a, lj.Settings([20.0] * 3, 2.5, 0.005, precision=1))])
+
+
+def test_no_device_is_a_loud_error(lj):
+    from virial_particles import _lib
+
+    n = C.c_int()
+    dev = 0
+    try:
+        import torch  # noqa: F401 -- only to count devices
+
+        dev = torch.cuda.device_count()
+    except Exception:
+        dev = 0
+    if dev:
+        pytest.skip("a GPU is present")
+    pos = np.random.default_rng(1).uniform(0, 20, (10, 3))
+    with pytest.raises(lj.LennardJonesError, match="no HIP device") as e:
+        lj.LennardJones((pos, np.zeros_like(pos)), lj.Settings([20.0] * 3, 2.5, 0.005))
+    assert e.value.code == _lib.E_NO_DEVICE
+    del n
+
+
+def test_host_lattices_match_the_oracle(oracle, lj):
+    a = lj.lattice_array(1000, 11.0, 1.0, 3)
+    p, v = oracle.lattice(1000, 11.0, 1.0, 3)
+    assert np.array_equal(a["pos"], p) and np.array_equal(a["vel"], v)
+    b = lj.lattice_box_array(1000, [9.0, 10.5, 14.25], 1.0, 3)
+    p, v = oracle.lattice_box(1000, [9.0, 10.5, 14.25], 1.0, 3)
+    assert np.array_equal(b["pos"], p) and np.array_equal(b["vel"], v)
+    f, side = lj.fcc_array(5, 0.8442, 0.722, 7)
+    p, v, s = oracle.fcc(5, 0.8442, 0.722, 7)
+    assert np.array_equal(f["pos"], p) and np.array_equal(f["vel"], v) and side == s
+
+
+def test_lattice_api_mirror(lj):
+    ps = lj.lattice(8, 4.0, 1.0, 1)
+    assert len(ps) == 8 and isinstance(ps[0], lj.Particle)
+    assert ps[0].pos == [1.0, 1.0, 1.0]
+
+
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 54, 4096, 5407, 65536])
+def test_xcd_swizzle_is_a_bijection(n):
+    from virial_particles import _lib
+
+    f = _lib.lib().vlj_debug_swizzle
+    m = [f(b, n) for b in range(n)]
+    assert sorted(m) == list(range(n))
+    # each XCD (b % 8) gets one contiguous share
+    for x in range(8):
+        share = sorted(m[b] for b in range(x, n, 8))
+        assert share == list(range(share[0], share[0] + len(share))) if share else True
+
+
+def test_measurement_helpers(lj):
+    m = lj.Measurement(0, 0.0, 3.0, -5.0, 2.0, [0, 0, 0], 10, 8.0, 1, 0, 0.25, 0)
+    assert m.energy() == -2.0
+    assert m.temperature(6.0) == 1.0
+    assert m.pressure() == (6.0 + 2.0) / 24.0
+    assert m.extended_energy() == -1.75

@@ -20,6 +20,11 @@ namespace vlj {
 constexpr int kBlock = 256;       // threads per atom block (4 wave64)
 constexpr int kScanCells = 1024;  // cells per scan block (256 threads x 4)
 constexpr int kSortCells = 4;     // cells per sort block (one wave64 per cell)
+constexpr int kBrickMax = 6;      // home cells of a brick along x and along y, at most
+constexpr int kStageCells = 3 * (kBrickMax + 2) * (kBrickMax + 2);  // staged cells, at most
+constexpr int kHomeCells = kBrickMax * kBrickMax;                    // home cells, at most
+constexpr int kStageMax = 2048;   // atoms a brick workgroup stages in LDS
+constexpr int kListStride = 64;   // lists are transposed per block of 64 atoms (one wave)
 constexpr int kSlots = 8;         // fixed-point slots per replica
 constexpr double kNearR2 = 2.25;  // condensed: neighbours within 1.5 sigma (lennard_jones.wgsl:417)
 
@@ -33,6 +38,7 @@ struct Replica {
   uint32_t atom_off, count;
   uint32_t cell_off, total_cells;
   uint32_t scan_block_off, scan_blocks;
+  uint32_t atom_block_off, atom_blocks;  // this replica's entries of the atom-block table
   uint32_t capacity;
   uint32_t thermostat;  // 0 none, 1 Langevin (BAOAB)
   uint64_t list_off;    // first neighbour entry of the replica
@@ -62,6 +68,13 @@ struct Range {
   uint32_t rep, first, end, pad;
 };
 
+// A brick: gx x gy home cells of one z layer of a replica's grid, from the
+// cell (x0, y0, z) = cell0 (index within the replica). Its workgroup stages
+// the 3 x (gy + 2) x (gx + 2) cells around it.
+struct Brick {
+  uint32_t rep, cell0, gx, gy;
+};
+
 // Everything a kernel touches, passed by value.
 struct Buffers {
   const Replica* rep;
@@ -69,7 +82,9 @@ struct Buffers {
   const Range* atom_blocks;
   const Range* scan_blocks;
   const Range* sort_blocks;
-  uint32_t n_atom_blocks, n_scan_blocks, n_sort_blocks, n_replicas;
+  const Brick* bricks;        // list build, forces and measurements
+  uint32_t n_atom_blocks, n_scan_blocks, n_sort_blocks, n_bricks, n_replicas;
+  unsigned long long* bmax;   // per atom block: max moved^2 (FP64 bits)
   double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;           // state, slot order
   double *x2, *y2, *z2, *vx2, *vy2, *vz2, *ax2, *ay2, *az2;  // reorder target
   double *xb, *yb, *zb;                                      // position at last list build
@@ -78,18 +93,46 @@ struct Buffers {
   uint32_t *unsorted, *sorted;                               // slots in cell order
   uint32_t *cell_count, *cell_cursor, *cell_scan, *cell_start, *cell_n;
   uint32_t *bsum, *bprefix;                                  // per scan block
-  uint32_t *nbr_count, *nbr;                                 // transposed lists
+  uint32_t* nbr_count;                                       // list length of each slot
+  uint16_t* nbr;                                             // lists of staged indices
   long long* slots;                                          // kSlots per replica
   uint32_t* tickets;                                         // last-block counters
 };
 
-// ---- R1: nearest periodic image. The image shift k = rint(d / L) is
-// taken off with one rounding, fma(-k, L, d) = d - kL; kL is exact (L has
-// 24 significant bits), so this equals the compare form d > L/2 -> d - L,
-// d < -L/2 -> d + L for every |d| away from L/2 (lennard_jones.wgsl:198-202).
-__device__ __forceinline__ double nearest(double d, double side, double inv) {
-  double k = __builtin_rint(d * inv);
-  return __builtin_fma(-k, side, d);
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
+// XCD b % 8). This maps them so that each XCD gets one contiguous share of
+// the table, whose atoms and neighbours then stay in that XCD's L2.
+__host__ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t n) {
+  const uint32_t q = n / 8u, r = n % 8u, xcd = b % 8u, idx = b / 8u;
+  return xcd < r ? xcd * (q + 1u) + idx : r * (q + 1u) + (xcd - r) * q + idx;
+}
+
+// A list entry is the neighbour's index in its brick's staging order (the
+// staged cells in (z, y, x) order, each cell's slots in cell order), 16 bits.
+// Lists are stored in octets of 8 entries (16 bytes), transposed within
+// blocks of 64 consecutive atoms (one wave): octet o of the replica's atom li
+// is at (u16 units)
+//   list_off + (li / 64) * octets * 512 + o * 512 + (li % 64) * 8,
+// octets = ceil(capacity / 8). A wave reads octet o of its 64 atoms as 1 KB
+// of contiguous memory; all of a wave's lists lie in octets KB.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ __forceinline__ uint32_t list_octets(uint32_t capacity) { return (capacity + 7u) / 8u; }
+
+__host__ __device__ __forceinline__ uint64_t list_base(const Replica& R, uint32_t li) {
+  return R.list_off + (uint64_t)(li / kListStride) * list_octets(R.capacity) * (8u * kListStride) +
+         (uint64_t)(li % kListStride) * 8u;
+}
+
+// u16 offset of entry k from list_base.
+__host__ __device__ __forceinline__ uint64_t list_entry(uint32_t k) {
+  return (uint64_t)(k / 8u) * (8u * kListStride) + (k % 8u);
+}
+
+// R1: back into the box by one length (positions are at most about a skin
+// outside it when the lists are rebuilt; lennard_jones.wgsl:188-196).
+__host__ __device__ __forceinline__ double wrap_one(double x, double side) {
+  return x >= side ? x - side : (x < 0.0 ? x + side : x);
 }
 
 __device__ __forceinline__ double dist2(double dx, double dy, double dz) {

@@ -69,16 +69,14 @@ __device__ __forceinline__ bool last_block(uint32_t* ticket) {
 // Rebuild decision (lennard_jones.wgsl:278-307), one thread per replica.
 // ---------------------------------------------------------------------------
 
-__device__ void decide(const Buffers& B, uint32_t r, bool advance_step) {
+__device__ void decide_from(const Buffers& B, uint32_t r, double moved2, bool advance_step) {
   Control* c = B.ctl + r;
-  const Replica& R = B.rep[r];
-  unsigned long long bits = atomicExch(&c->max_moved2, 0ull);
-  double moved2 = __longlong_as_double((long long)bits);
+  const Replica R = B.rep[r];
   uint32_t built = c->built;
   uint32_t rebuilds = c->rebuilds;
   bool rebuild = !built || moved2 > R.half_skin2;
   c->rebuild = rebuild ? 1u : 0u;
-  c->reorder = (rebuild && (rebuilds % 4u) == 0u) ? 1u : 0u;
+  c->reorder = rebuild ? 1u : 0u;  // R8: every rebuild moves atoms into cell order
   if (rebuild) {
     c->rebuilds = rebuilds + 1u;
     c->built = 1u;
@@ -86,8 +84,26 @@ __device__ void decide(const Buffers& B, uint32_t r, bool advance_step) {
   if (advance_step) c->step = c->step + 1ull;
 }
 
-__global__ void __launch_bounds__(kBlock) k_decide(Buffers B) {
-  for (uint32_t r = threadIdx.x; r < B.n_replicas; r += blockDim.x) decide(B, r, false);
+// One workgroup per replica: the largest moved^2 over the replica's drift
+// workgroups (bmax), then the decision. `advance` also counts the step.
+__global__ void __launch_bounds__(kBlock) k_decide(Buffers B, int advance) {
+  const uint32_t r = blockIdx.x;
+  const Replica R = B.rep[r];
+  unsigned long long m = 0ull;
+  if (advance)
+    for (uint32_t b = R.atom_block_off + threadIdx.x; b < R.atom_block_off + R.atom_blocks; b += kBlock) {
+      const unsigned long long v = B.bmax[b];
+      m = v > m ? v : m;
+    }
+  __shared__ unsigned long long wmax[kBlock / 64];
+  m = wave_max_u64(m);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; w++) m = wmax[w] > m ? wmax[w] : m;
+    decide_from(B, r, __longlong_as_double((long long)m), advance != 0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -97,8 +113,9 @@ __global__ void __launch_bounds__(kBlock) k_decide(Buffers B) {
 // LANGEVIN: some replica of the launch uses BAOAB (others stay NVE).
 template <bool LANGEVIN>
 __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
-  const Range blk = B.atom_blocks[blockIdx.x];
-  const Replica& R = B.rep[blk.rep];
+  const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const Range blk = B.atom_blocks[b];
+  const Replica R = B.rep[blk.rep];
   const uint32_t i = blk.first + threadIdx.x;
   unsigned long long moved = 0ull;
   if (i < blk.end) {
@@ -121,15 +138,14 @@ __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
       } else {
         x = __builtin_fma(v, R.dt, x);
       }
-      // wrap by one box length (lennard_jones.wgsl:188-196)
-      x = x >= R.box[k] ? x - R.box[k] : (x < 0.0 ? x + R.box[k] : x);
       V[k][i] = v;
-      X[k][i] = x;
-      d[k] = nearest(x - XB[k][i], R.box[k], R.inv_box[k]);
+      X[k][i] = x;  // wrapped at the next rebuild (R1)
+      d[k] = x - XB[k][i];
     }
     moved = (unsigned long long)__double_as_longlong(dist2(d[0], d[1], d[2]));
   }
-  // one global atomic per workgroup (lennard_jones.wgsl:215-220)
+  // One value per workgroup, no atomics (lennard_jones.wgsl:215-220 uses
+  // one global atomicMax per workgroup); k_decide reduces them.
   __shared__ unsigned long long wmax[kBlock / 64];
   moved = wave_max_u64(moved);
   if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = moved;
@@ -138,11 +154,7 @@ __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
     unsigned long long m = wmax[0];
 #pragma unroll
     for (int w = 1; w < kBlock / 64; w++) m = wmax[w] > m ? wmax[w] : m;
-    atomicMax(&B.ctl[blk.rep].max_moved2, m);
-  }
-  if (last_block(B.tickets + 0)) {
-    for (uint32_t r = threadIdx.x; r < B.n_replicas; r += blockDim.x) decide(B, r, true);
-    if (threadIdx.x == 0) atomicExch(B.tickets + 0, 0u);
+    B.bmax[b] = m;
   }
 }
 
@@ -152,12 +164,17 @@ __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(kBlock) k_bin(Buffers B) {
-  const Range blk = B.atom_blocks[blockIdx.x];
+  const Range blk = B.atom_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
   if (!B.ctl[blk.rep].rebuild) return;
-  const Replica& R = B.rep[blk.rep];
+  const Replica R = B.rep[blk.rep];
   const uint32_t i = blk.first + threadIdx.x;
   if (i >= blk.end) return;
-  uint32_t c = R.cell_off + cell_local(R, B.x[i], B.y[i], B.z[i]);
+  // R1: positions go back into the box when the lists are rebuilt
+  const double x = wrap_one(B.x[i], R.box[0]), y = wrap_one(B.y[i], R.box[1]), z = wrap_one(B.z[i], R.box[2]);
+  B.x[i] = x;
+  B.y[i] = y;
+  B.z[i] = z;
+  uint32_t c = R.cell_off + cell_local(R, x, y, z);
   B.cell_of[i] = c;
   atomicAdd(B.cell_count + c, 1u);
 }
@@ -201,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(Buffers B) {
   if (last_block(B.tickets + 1)) {
     for (uint32_t r = threadIdx.x; r < B.n_replicas; r += blockDim.x) {
       if (!B.ctl[r].rebuild) continue;
-      const Replica& R = B.rep[r];
+      const Replica R = B.rep[r];
       uint32_t acc = 0;
       for (uint32_t b = R.scan_block_off; b < R.scan_block_off + R.scan_blocks; b++) {
         B.bprefix[b] = acc;
@@ -218,9 +235,9 @@ __device__ __forceinline__ uint32_t cell_first_slot(const Buffers& B, const Repl
 }
 
 __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
-  const Range blk = B.atom_blocks[blockIdx.x];
+  const Range blk = B.atom_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
   if (!B.ctl[blk.rep].rebuild) return;
-  const Replica& R = B.rep[blk.rep];
+  const Replica R = B.rep[blk.rep];
   const uint32_t i = blk.first + threadIdx.x;
   if (i >= blk.end) return;
   uint32_t c = B.cell_of[i];
@@ -232,9 +249,9 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
 // in the cell, which puts the cell in ascending slot order whatever order
 // the scatter produced (the result of grid.wgsl:129-148's insertion sort).
 __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
-  const Range blk = B.sort_blocks[blockIdx.x];
+  const Range blk = B.sort_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
   if (!B.ctl[blk.rep].rebuild) return;
-  const Replica& R = B.rep[blk.rep];
+  const Replica R = B.rep[blk.rep];
   const uint32_t c = blk.first + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= blk.end) return;
@@ -264,7 +281,7 @@ __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
 // Slot t takes the atom at sorted[t] (lennard_jones.wgsl:419-452); the copy
 // goes to the second state set and `build` copies it back.
 __global__ void __launch_bounds__(kBlock) k_reorder(Buffers B) {
-  const Range blk = B.atom_blocks[blockIdx.x];
+  const Range blk = B.atom_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
   if (!B.ctl[blk.rep].reorder) return;
   const uint32_t t = blk.first + threadIdx.x;
   if (t >= blk.end) return;
@@ -283,70 +300,283 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Buffers B) {
   B.sorted[t] = t;
 }
 
-// Verlet list of each atom: the 27 cells around its own, dz outermost,
-// slots in cell order, every j != i within rc + skin (lennard_jones.wgsl:454-499).
-// Entry k of atom i sits at list_off + k * count + (i - atom_off).
-__global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
-  const Range blk = B.atom_blocks[blockIdx.x];
-  const Control& ctl = B.ctl[blk.rep];
-  if (!ctl.rebuild) return;
-  const bool moved = ctl.reorder != 0u;
-  const Replica& R = B.rep[blk.rep];
-  const uint32_t i = blk.first + threadIdx.x;
-  if (i >= blk.end) return;
-  const double* X = moved ? B.x2 : B.x;
-  const double* Y = moved ? B.y2 : B.y;
-  const double* Z = moved ? B.z2 : B.z;
-  const double xi = X[i], yi = Y[i], zi = Z[i];
-  uint32_t cg = moved ? B.cell_of2[i] : B.cell_of[i];
-  if (moved) {
-    B.x[i] = xi;
-    B.y[i] = yi;
-    B.z[i] = zi;
-    B.vx[i] = B.vx2[i];
-    B.vy[i] = B.vy2[i];
-    B.vz[i] = B.vz2[i];
-    B.ax[i] = B.ax2[i];
-    B.ay[i] = B.ay2[i];
-    B.az[i] = B.az2[i];
-    B.orig[i] = B.orig2[i];
-    B.cell_of[i] = cg;
+// ---------------------------------------------------------------------------
+// Bricks: staging of a brick's neighbourhood in LDS
+// ---------------------------------------------------------------------------
+
+// LDS of a brick workgroup: the staged cells (first slot, staged offset,
+// periodic image) and, when they fit, the staged positions with the image
+// added, so a displacement is one subtraction (R4, R5). Every rebuild puts
+// the atoms in cell order (R8), so staged cell c holds the slots
+// start[c] .. start[c] + (off[c + 1] - off[c]).
+struct Stage {
+  uint32_t start[kStageCells];
+  uint32_t off[kStageCells + 1];
+  uint32_t hoff[kHomeCells + 1];  // home atoms numbered home cell by home cell
+  uint8_t image[kStageCells];     // (ix + 1) + 3 (iy + 1) + 9 (iz + 1), i in {-1, 0, 1}
+  uint32_t ns;                    // staged cells
+  uint32_t total;                 // staged atoms
+  double x[kStageMax], y[kStageMax], z[kStageMax];
+};
+
+// Staged index of the cell at offset (dz, yy, xx) from the brick's corner
+// (-1, -1, -1): dz in 0..2, yy in 0..gy+1, xx in 0..gx+1.
+__device__ __forceinline__ uint32_t staged_cell(const Brick& g, uint32_t dz, uint32_t yy, uint32_t xx) {
+  return (dz * (g.gy + 2u) + yy) * (g.gx + 2u) + xx;
+}
+
+// Staged cell of staged atom e (binary search over the staged offsets).
+__device__ __forceinline__ uint32_t stage_cell_of(const Stage& S, uint32_t e) {
+  uint32_t lo = 0, hi = S.ns - 1u;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1u) >> 1;
+    if (S.off[mid] <= e) lo = mid; else hi = mid - 1u;
   }
+  return lo;
+}
+
+__device__ __forceinline__ void image_shift(const Replica& R, uint32_t code, double& sx, double& sy, double& sz) {
+  sx = (double)((int)(code % 3u) - 1) * R.box[0];
+  sy = (double)((int)((code / 3u) % 3u) - 1) * R.box[1];
+  sz = (double)((int)(code / 9u) - 1) * R.box[2];
+}
+
+// Fills S for brick g from the positions X, Y, Z. Every thread of the
+// workgroup calls it. Returns whether the positions were staged (the
+// neighbourhood holds at most kStageMax atoms).
+__device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, const double* X, const double* Y,
+                            const double* Z, Stage& S) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
-  const int c = (int)(cg - R.cell_off);
-  const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-  const uint32_t cap = R.capacity;
-  uint32_t* out = B.nbr + R.list_off + (i - R.atom_off);
-  const uint32_t stride = R.count;
-  uint32_t found = 0;
-  for (int dz = -1; dz <= 1; dz++) {
-    const int ncz = (cz + dz + nz) % nz;
-    for (int dy = -1; dy <= 1; dy++) {
-      const int ncy = (cy + dy + ny) % ny;
-      for (int dx = -1; dx <= 1; dx++) {
-        const int ncx = (cx + dx + nx) % nx;
-        const uint32_t cell = R.cell_off + (uint32_t)((ncz * ny + ncy) * nx + ncx);
-        const uint32_t s0 = B.cell_start[cell];
-        const uint32_t s1 = s0 + B.cell_n[cell];
-        for (uint32_t s = s0; s < s1; s++) {
-          const uint32_t j = B.sorted[s];
-          if (j == i) continue;
-          double ddx = nearest(X[j] - xi, R.box[0], R.inv_box[0]);
-          double ddy = nearest(Y[j] - yi, R.box[1], R.inv_box[1]);
-          double ddz = nearest(Z[j] - zi, R.box[2], R.inv_box[2]);
-          if (dist2(ddx, ddy, ddz) < R.list_r2) {
-            if (found < cap) out[(size_t)found * stride] = j;
-            found++;
-          }
-        }
+  const int c0 = (int)g.cell0;
+  const int cx0 = c0 % nx, cy0 = (c0 / nx) % ny, cz = c0 / (nx * ny);
+  const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
+  const uint32_t ns = 3u * sy * sx;
+  const uint32_t t = threadIdx.x;
+  uint32_t n_t = 0;
+  if (t < ns) {
+    const int xx = (int)(t % sx), yy = (int)((t / sx) % sy), dz = (int)(t / (sx * sy));
+    const int ux = cx0 - 1 + xx, uy = cy0 - 1 + yy, uz = cz - 1 + dz;  // unwrapped
+    const int cx = (ux + nx) % nx, cy = (uy + ny) % ny, ccz = (uz + nz) % nz;
+    const uint32_t cell = R.cell_off + (uint32_t)((ccz * ny + cy) * nx + cx);
+    S.start[t] = B.cell_start[cell];
+    n_t = B.cell_n[cell];
+    // image of the cell seen from the brick (R4)
+    const int ix = ux < 0 ? -1 : (ux >= nx ? 1 : 0);
+    const int iy = uy < 0 ? -1 : (uy >= ny ? 1 : 0);
+    const int iz = uz < 0 ? -1 : (uz >= nz ? 1 : 0);
+    S.image[t] = (uint8_t)((ix + 1) + 3 * (iy + 1) + 9 * (iz + 1));
+  }
+  // exclusive scan of the staged counts over the workgroup
+  __shared__ uint32_t wtot[kBlock / 64];
+  const int lane = t & 63, wave = t >> 6;
+  const uint32_t incl = wave_incl_scan_u32(n_t, lane);
+  if (lane == 63) wtot[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < wave; w++) before += wtot[w];
+  if (t < ns) S.off[t] = before + incl - n_t;
+  if (t == ns - 1u) {
+    S.off[ns] = before + incl;
+    S.total = before + incl;
+    S.ns = ns;
+  }
+  __syncthreads();
+  if (t == 0) {  // home atoms, home cell by home cell (hy outer, hx inner)
+    uint32_t run = 0;
+    for (uint32_t h = 0; h < g.gx * g.gy; h++) {
+      S.hoff[h] = run;
+      const uint32_t hs = staged_cell(g, 1u, h / g.gx + 1u, h % g.gx + 1u);
+      run += S.off[hs + 1u] - S.off[hs];
+    }
+    S.hoff[g.gx * g.gy] = run;
+  }
+  const uint32_t total = S.total;
+  const bool staged = total <= (uint32_t)kStageMax;
+  if (staged) {
+    // every load of this thread's entries first, then the LDS stores
+    constexpr int kPer = kStageMax / kBlock;
+    uint32_t code[kPer];
+    double px[kPer], py[kPer], pz[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t e = t + (uint32_t)k * kBlock;
+      code[k] = 13u;
+      px[k] = py[k] = pz[k] = 0.0;
+      if (e < total) {
+        const uint32_t c = stage_cell_of(S, e);
+        const uint32_t j = S.start[c] + (e - S.off[c]);
+        code[k] = S.image[c];
+        px[k] = X[j];
+        py[k] = Y[j];
+        pz[k] = Z[j];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t e = t + (uint32_t)k * kBlock;
+      if (e < total) {
+        double shx, shy, shz;
+        image_shift(R, code[k], shx, shy, shz);
+        S.x[e] = px[k] + shx;
+        S.y[e] = py[k] + shy;
+        S.z[e] = pz[k] + shz;
       }
     }
   }
-  B.nbr_count[i] = found < cap ? found : cap;
-  if (found > cap) atomicMax(&B.ctl[blk.rep].overflow, found);
-  B.xb[i] = xi;
-  B.yb[i] = yi;
-  B.zb[i] = zi;
+  __syncthreads();
+  return staged;
+}
+
+// Home atom h of the brick: its home cell, staged index and slot.
+struct Home {
+  uint32_t hx, hy, e, slot;
+};
+
+__device__ __forceinline__ Home home_atom(const Brick& g, const Stage& S, uint32_t h) {
+  uint32_t hc = 0;
+  while (hc + 1u < g.gx * g.gy && S.hoff[hc + 1u] <= h) hc++;
+  Home a;
+  a.hx = hc % g.gx;
+  a.hy = hc / g.gx;
+  const uint32_t hs = staged_cell(g, 1u, a.hy + 1u, a.hx + 1u);
+  const uint32_t idx = h - S.hoff[hc];
+  a.e = S.off[hs] + idx;
+  a.slot = S.start[hs] + idx;
+  return a;
+}
+
+// Position of staged atom e, with its image: from LDS when staged, else
+// looked up through the cell tables in global memory.
+__device__ __forceinline__ void staged_pos(const Replica& R, const Stage& S, bool staged, uint32_t e,
+                                           const double* X, const double* Y, const double* Z, double& x,
+                                           double& y, double& z) {
+  if (staged) {
+    x = S.x[e];
+    y = S.y[e];
+    z = S.z[e];
+  } else {
+    const uint32_t c = stage_cell_of(S, e);
+    const uint32_t j = S.start[c] + (e - S.off[c]);
+    double shx, shy, shz;
+    image_shift(R, S.image[c], shx, shy, shz);
+    x = X[j] + shx;
+    y = Y[j] + shy;
+    z = Z[j] + shz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Verlet lists (lennard_jones.wgsl:454-499)
+// ---------------------------------------------------------------------------
+
+// Appends one atom's list entries eight at a time (one 16-byte store).
+struct ListWriter {
+  uint16_t* base;  // list_base of the atom
+  u32x4 buf;
+  uint32_t found, cap;
+  __device__ __forceinline__ void push(uint32_t e) {
+    if (found < cap) {
+      const uint32_t s = found & 7u, w = s >> 1;
+      const uint32_t word = (w == 0u ? buf.x : (w == 1u ? buf.y : (w == 2u ? buf.z : buf.w)));
+      const uint32_t merged = (s & 1u) ? ((word & 0xffffu) | (e << 16)) : ((word & 0xffff0000u) | e);
+      buf.x = w == 0u ? merged : buf.x;
+      buf.y = w == 1u ? merged : buf.y;
+      buf.z = w == 2u ? merged : buf.z;
+      buf.w = w == 3u ? merged : buf.w;
+      if (s == 7u) *(u32x4*)(base + list_entry(found & ~7u)) = buf;
+    }
+    found++;
+  }
+  __device__ __forceinline__ void finish() {
+    const uint32_t stored = found < cap ? found : cap;
+    if (stored & 7u) *(u32x4*)(base + list_entry(stored & ~7u)) = buf;
+  }
+};
+
+// One workgroup per brick. Each home atom walks the 27 cells around its
+// own, dz outermost, slots in cell order, and lists every other atom
+// within rc + skin by its staged index. On a reorder the workgroup also
+// copies its home atoms' state back from the second set.
+__global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
+  const Brick g = B.bricks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const Control& ctl = B.ctl[g.rep];
+  if (!ctl.rebuild) return;
+  const bool moved = ctl.reorder != 0u;
+  const Replica R = B.rep[g.rep];
+  const double* X = moved ? B.x2 : B.x;
+  const double* Y = moved ? B.y2 : B.y;
+  const double* Z = moved ? B.z2 : B.z;
+  __shared__ Stage S;
+  const bool staged = stage_brick(B, R, g, X, Y, Z, S);
+  const uint32_t total = S.total;
+  const uint32_t cap = R.capacity;
+  const uint32_t H = S.hoff[g.gx * g.gy];
+  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+    const Home a = home_atom(g, S, h);
+    const uint32_t i = a.slot;
+    double xi, yi, zi;
+    staged_pos(R, S, staged, a.e, X, Y, Z, xi, yi, zi);
+    if (moved) {  // copy the reordered state back into the main set
+      B.x[i] = xi;
+      B.y[i] = yi;
+      B.z[i] = zi;
+      B.vx[i] = B.vx2[i];
+      B.vy[i] = B.vy2[i];
+      B.vz[i] = B.vz2[i];
+      B.ax[i] = B.ax2[i];
+      B.ay[i] = B.ay2[i];
+      B.az[i] = B.az2[i];
+      B.orig[i] = B.orig2[i];
+      B.cell_of[i] = B.cell_of2[i];
+    }
+    ListWriter list = {B.nbr + list_base(R, i - R.atom_off), u32x4{0u, 0u, 0u, 0u}, 0u, cap};
+    if (total <= 0xffffu) {
+      for (uint32_t dz = 0; dz < 3u; dz++)
+        for (uint32_t dy = 0; dy < 3u; dy++)
+          for (uint32_t dx = 0; dx < 3u; dx++) {
+            const uint32_t sc = staged_cell(g, dz, a.hy + dy, a.hx + dx);
+            const uint32_t e0 = S.off[sc], e1 = S.off[sc + 1u];
+            if (staged) {
+              // four candidates per pass: all LDS reads first, then the
+              // appends in order
+              for (uint32_t e2 = e0; e2 < e1; e2 += 4u) {
+                bool in[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                  const uint32_t q = e2 + (uint32_t)c;
+                  const uint32_t qq = q < e1 ? q : e0;
+                  const double ddx = S.x[qq] - xi, ddy = S.y[qq] - yi, ddz = S.z[qq] - zi;
+                  // q == a.e is the atom itself (its home copy, unshifted)
+                  in[c] = q < e1 && q != a.e && dist2(ddx, ddy, ddz) < R.list_r2;
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                  if (in[c]) list.push(e2 + (uint32_t)c);
+              }
+            } else {
+              double shx, shy, shz;
+              image_shift(R, S.image[sc], shx, shy, shz);
+              for (uint32_t e2 = e0; e2 < e1; e2++) {
+                if (e2 == a.e) continue;
+                const uint32_t j = S.start[sc] + (e2 - e0);
+                const double ddx = (X[j] + shx) - xi;
+                const double ddy = (Y[j] + shy) - yi;
+                const double ddz = (Z[j] + shz) - zi;
+                if (dist2(ddx, ddy, ddz) < R.list_r2) list.push(e2);
+              }
+            }
+          }
+      list.finish();
+    } else {
+      list.found = total;  // staged indices would not fit in 16 bits
+    }
+    const uint32_t found = list.found;
+    B.nbr_count[i] = found < cap ? found : 0u;
+    if (found > cap) atomicMax(&B.ctl[g.rep].overflow, found);
+    B.xb[i] = xi;
+    B.yb[i] = yi;
+    B.zb[i] = zi;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -416,39 +646,73 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
   }
 }
 
+constexpr int kOctetBatch = 4;  // list octets (32 entries) in flight per batch
+
+__device__ __forceinline__ u32x4 load_octet(const uint16_t* base, uint32_t o, uint32_t no) {
+  if (o < no) return __builtin_nontemporal_load((const u32x4*)(base + (uint64_t)o * (8u * kListStride)));
+  return u32x4{0u, 0u, 0u, 0u};
+}
+
+// Atom i's list in order, kOctetBatch octets at a time, double-buffered:
+// the next batch's loads are in flight while this one is summed. Positions
+// come from the brick's LDS.
 template <int PREC, bool SF, bool ENERGIES>
-__device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica& R, uint32_t i) {
-  const double xi = B.x[i], yi = B.y[i], zi = B.z[i];
+__device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica& R, const Stage& S, bool staged,
+                                                uint32_t i, double xi, double yi, double zi) {
   const uint32_t cnt = B.nbr_count[i];
-  const uint32_t* lst = B.nbr + R.list_off + (i - R.atom_off);
-  const uint32_t stride = R.count;
+  const uint32_t no = (cnt + 7u) / 8u;
+  const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
   PairAcc a = {0.0, 0.0, 0.0, 0.0, 0.0, 0u, 0u};
-  for (uint32_t k = 0; k < cnt; k++) {
-    const uint32_t j = lst[(size_t)k * stride];
-    const double dx = nearest(B.x[j] - xi, R.box[0], R.inv_box[0]);
-    const double dy = nearest(B.y[j] - yi, R.box[1], R.inv_box[1]);
-    const double dz = nearest(B.z[j] - zi, R.box[2], R.inv_box[2]);
-    const double r2 = dist2(dx, dy, dz);
-    if (r2 < R.rc2) pair_accumulate<PREC, SF, ENERGIES>(R, dx, dy, dz, r2, a);
+  u32x4 cur[kOctetBatch], nxt[kOctetBatch];
+#pragma unroll
+  for (int b = 0; b < kOctetBatch; b++) cur[b] = load_octet(lst, (uint32_t)b, no);
+  for (uint32_t o0 = 0; o0 < no; o0 += kOctetBatch) {
+#pragma unroll
+    for (int b = 0; b < kOctetBatch; b++) nxt[b] = load_octet(lst, o0 + kOctetBatch + (uint32_t)b, no);
+#pragma unroll
+    for (int b = 0; b < kOctetBatch; b++) {
+      const uint32_t w[4] = {cur[b].x, cur[b].y, cur[b].z, cur[b].w};
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        if ((o0 + (uint32_t)b) * 8u + (uint32_t)c < cnt) {
+          const uint32_t e = (w[c >> 1] >> ((c & 1) * 16)) & 0xffffu;
+          double xj, yj, zj;
+          staged_pos(R, S, staged, e, B.x, B.y, B.z, xj, yj, zj);
+          const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+          const double r2 = dist2(dx, dy, dz);
+          if (r2 < R.rc2) pair_accumulate<PREC, SF, ENERGIES>(R, dx, dy, dz, r2, a);
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < kOctetBatch; b++) cur[b] = nxt[b];
   }
   return a;
 }
 
-// New accelerations and (kick) the second half kick v += a dt/2.
+// New accelerations and (kick) the second half kick v += a dt/2, one
+// workgroup per brick.
 template <int PREC, bool SF>
 __global__ void __launch_bounds__(kBlock) k_force(Buffers B, int kick) {
-  const Range blk = B.atom_blocks[blockIdx.x];
-  const Replica& R = B.rep[blk.rep];
-  const uint32_t i = blk.first + threadIdx.x;
-  if (i >= blk.end) return;
-  const PairAcc a = interactions<PREC, SF, false>(B, R, i);
-  B.ax[i] = a.ax;
-  B.ay[i] = a.ay;
-  B.az[i] = a.az;
-  if (kick) {
-    B.vx[i] = __builtin_fma(a.ax, R.half_dt, B.vx[i]);
-    B.vy[i] = __builtin_fma(a.ay, R.half_dt, B.vy[i]);
-    B.vz[i] = __builtin_fma(a.az, R.half_dt, B.vz[i]);
+  const Brick g = B.bricks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const Replica R = B.rep[g.rep];
+  __shared__ Stage S;
+  const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+  const uint32_t H = S.hoff[g.gx * g.gy];
+  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+    const Home at = home_atom(g, S, h);
+    const uint32_t i = at.slot;
+    double xi, yi, zi;
+    staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
+    const PairAcc a = interactions<PREC, SF, false>(B, R, S, staged, i, xi, yi, zi);
+    B.ax[i] = a.ax;
+    B.ay[i] = a.ay;
+    B.az[i] = a.az;
+    if (kick) {
+      B.vx[i] = __builtin_fma(a.ax, R.half_dt, B.vx[i]);
+      B.vy[i] = __builtin_fma(a.ay, R.half_dt, B.vy[i]);
+      B.vz[i] = __builtin_fma(a.az, R.half_dt, B.vz[i]);
+    }
   }
 }
 
@@ -478,24 +742,30 @@ __device__ __forceinline__ void block_add_slots(long long v[kSlots], long long* 
 
 template <int PREC, bool SF>
 __global__ void __launch_bounds__(kBlock) k_measure(Buffers B) {
-  const Range blk = B.atom_blocks[blockIdx.x];
-  const Replica& R = B.rep[blk.rep];
-  const uint32_t i = blk.first + threadIdx.x;
+  const Brick g = B.bricks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const Replica R = B.rep[g.rep];
+  __shared__ Stage S;
+  const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+  const uint32_t H = S.hoff[g.gx * g.gy];
   long long v[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (i < blk.end) {
-    const PairAcc a = interactions<PREC, SF, true>(B, R, i);
+  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+    const Home at = home_atom(g, S, h);
+    const uint32_t i = at.slot;
+    double xi, yi, zi;
+    staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
+    const PairAcc a = interactions<PREC, SF, true>(B, R, S, staged, i, xi, yi, zi);
     const double vx = B.vx[i], vy = B.vy[i], vz = B.vz[i];
     const double ke = 0.5 * dist2(vx, vy, vz);
-    v[0] = to_fixed(ke, R.energy_fixed);
-    v[1] = to_fixed(a.pe, R.energy_fixed);
-    v[2] = to_fixed(a.w, R.energy_fixed);
-    v[3] = to_fixed(vx, R.momentum_fixed);
-    v[4] = to_fixed(vy, R.momentum_fixed);
-    v[5] = to_fixed(vz, R.momentum_fixed);
-    v[6] = a.near >= R.near_count ? 1 : 0;
-    v[7] = (long long)a.within;
+    v[0] += to_fixed(ke, R.energy_fixed);
+    v[1] += to_fixed(a.pe, R.energy_fixed);
+    v[2] += to_fixed(a.w, R.energy_fixed);
+    v[3] += to_fixed(vx, R.momentum_fixed);
+    v[4] += to_fixed(vy, R.momentum_fixed);
+    v[5] += to_fixed(vz, R.momentum_fixed);
+    v[6] += a.near >= R.near_count ? 1 : 0;
+    v[7] += (long long)a.within;
   }
-  block_add_slots(v, B.slots + (size_t)blk.rep * kSlots);
+  block_add_slots(v, B.slots + (size_t)g.rep * kSlots);
 }
 
 __global__ void __launch_bounds__(kBlock) k_zero_slots(Buffers B) {
@@ -523,8 +793,10 @@ __global__ void k_zero_i64(long long* p, uint32_t n) {
 
 static inline dim3 grid1(uint32_t blocks) { return dim3(blocks, 1, 1); }
 
+static inline dim3 decide_grid(const Buffers& B) { return dim3(B.n_replicas, 1, 1); }
+
 hipError_t launch_decide(const Buffers& B, hipStream_t s) {
-  hipLaunchKernelGGL(k_decide, dim3(1), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_decide, decide_grid(B), dim3(kBlock), 0, s, B, 0);
   return hipGetLastError();
 }
 
@@ -534,6 +806,7 @@ hipError_t launch_drift(const Buffers& B, bool langevin, hipStream_t s) {
     hipLaunchKernelGGL(k_drift<true>, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
   else
     hipLaunchKernelGGL(k_drift<false>, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_decide, decide_grid(B), dim3(kBlock), 0, s, B, 1);
   return hipGetLastError();
 }
 
@@ -544,23 +817,24 @@ hipError_t launch_rebuild(const Buffers& B, hipStream_t s) {
   hipLaunchKernelGGL(k_scatter, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_sort, grid1(B.n_sort_blocks), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_reorder, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
-  hipLaunchKernelGGL(k_build, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_build, grid1(B.n_bricks), dim3(kBlock), 0, s, B);
   return hipGetLastError();
 }
 
 hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s) {
   if (!B.n_atom_blocks) return hipSuccess;
+  const uint32_t nb = B.n_bricks;
   const int k = kick ? 1 : 0;
   if (precision == 0) {
     if (shifted)
-      hipLaunchKernelGGL((k_force<0, true>), grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<0, true>), grid1(nb), dim3(kBlock), 0, s, B, k);
     else
-      hipLaunchKernelGGL((k_force<0, false>), grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<0, false>), grid1(nb), dim3(kBlock), 0, s, B, k);
   } else {
     if (shifted)
-      hipLaunchKernelGGL((k_force<1, true>), grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<1, true>), grid1(nb), dim3(kBlock), 0, s, B, k);
     else
-      hipLaunchKernelGGL((k_force<1, false>), grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<1, false>), grid1(nb), dim3(kBlock), 0, s, B, k);
   }
   return hipGetLastError();
 }
@@ -571,14 +845,14 @@ hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStre
   if (!B.n_atom_blocks) return hipGetLastError();
   if (precision == 0) {
     if (shifted)
-      hipLaunchKernelGGL((k_measure<0, true>), grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<0, true>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
     else
-      hipLaunchKernelGGL((k_measure<0, false>), grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<0, false>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
   } else {
     if (shifted)
-      hipLaunchKernelGGL((k_measure<1, true>), grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<1, true>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
     else
-      hipLaunchKernelGGL((k_measure<1, false>), grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<1, false>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
   }
   return hipGetLastError();
 }

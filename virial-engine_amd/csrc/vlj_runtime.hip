@@ -38,6 +38,7 @@ struct Ring {
 struct HostReplica {
   vlj_settings settings;
   double energy_scale, momentum_scale;
+  uint32_t brick_x, brick_y;  // brick shape in cells
 };
 
 }  // namespace
@@ -49,6 +50,7 @@ struct vlj_sim {
   std::vector<Replica> rep;
   std::vector<HostReplica> host;
   std::vector<Range> atom_blocks, scan_blocks, sort_blocks;
+  std::vector<Brick> bricks;
   uint32_t n_total = 0, cells_total = 0;
   uint64_t list_total = 0;
   int precision = 0;
@@ -100,7 +102,7 @@ uint32_t host_cell_axis(double x, float cs, uint32_t n) {
 // capacity from the densest cell of the initial state (not only the box
 // average), and dt kept in FP64.
 int derive(const vlj_particle* p, size_t n, const vlj_settings& st, Replica& R, HostReplica& H,
-           std::string& err) {
+           std::vector<uint32_t>& occ, std::string& err) {
   std::memset(&R, 0, sizeof(R));
   H.settings = st;
   if (!(st.cutoff > 0.0) || !(st.skin >= 0.0) || !(st.dt > 0.0)) {
@@ -154,7 +156,7 @@ int derive(const vlj_particle* p, size_t n, const vlj_settings& st, Replica& R, 
   const double volume = R.box[0] * R.box[1] * R.box[2];
   const double density = (double)n / volume;
   const double sphere = 4.0 / 3.0 * 3.14159265358979323846 * list_radius * list_radius * list_radius;
-  std::vector<uint32_t> occ(R.total_cells, 0);
+  occ.assign(R.total_cells, 0);
   uint32_t occ_max = 0;
   for (size_t i = 0; i < n; i++) {
     double w[3];
@@ -290,6 +292,43 @@ int prime_if_needed(vlj_sim* s) {
   return VLJ_OK;
 }
 
+// Brick shape of a replica: the most home cells gx * gy (gx, gy <= kBrickMax
+// and within the grid) whose densest staged neighbourhood, counted on the
+// initial cell occupancy, holds at most 85% of kStageMax atoms; the margin
+// absorbs density fluctuations. A brick that still overflows runs a slower
+// path through global memory.
+void choose_brick(const Replica& R, const std::vector<uint32_t>& occ, uint32_t& bx, uint32_t& by) {
+  const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
+  const double limit = 0.85 * kStageMax;
+  bx = 1;
+  by = 1;
+  uint32_t best = 0;
+  for (int gy = 1; gy <= std::min(kBrickMax, ny); gy++)
+    for (int gx = gy; gx <= std::min(kBrickMax, nx); gx++) {
+      const uint32_t home = (uint32_t)(gx * gy);
+      if (home <= best) continue;
+      uint32_t worst = 0;
+      for (int z = 0; z < nz && worst <= limit; z++)
+        for (int y0 = 0; y0 < ny && worst <= limit; y0 += gy)
+          for (int x0 = 0; x0 < nx && worst <= limit; x0 += gx) {
+            uint32_t sum = 0;
+            const int ex = std::min(gx, nx - x0), ey = std::min(gy, ny - y0);
+            for (int dz = -1; dz <= 1; dz++)
+              for (int yy = -1; yy <= ey; yy++)
+                for (int xx = -1; xx <= ex; xx++) {
+                  const int cx = (x0 + xx + nx) % nx, cy = (y0 + yy + ny) % ny, cz = (z + dz + nz) % nz;
+                  sum += occ[(size_t)(cz * ny + cy) * nx + cx];
+                }
+            worst = std::max(worst, sum);
+          }
+      if (worst <= limit) {
+        best = home;
+        bx = (uint32_t)gx;
+        by = (uint32_t)gy;
+      }
+    }
+}
+
 // Builds a handle over `replicas` systems.
 int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings* settings,
            size_t replicas, vlj_sim** out) {
@@ -321,7 +360,8 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     Replica& R = s->rep[r];
     std::string err;
     size_t n = (size_t)(offsets[r + 1] - offsets[r]);
-    rc = derive(all + offsets[r], n, settings[r], R, s->host[r], err);
+    std::vector<uint32_t> occ;
+    rc = derive(all + offsets[r], n, settings[r], R, s->host[r], occ, err);
     if (rc != VLJ_OK) {
       delete s;
       return fail(nullptr, rc, replicas > 1 ? "replica " + std::to_string(r) + ": " + err : err);
@@ -332,6 +372,17 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     R.scan_block_off = scan_off;
     R.scan_blocks = (R.total_cells + kScanCells - 1) / kScanCells;
     R.list_off = list_off;
+    R.atom_block_off = (uint32_t)s->atom_blocks.size();
+    R.atom_blocks = (R.count + kBlock - 1) / kBlock;
+    choose_brick(R, occ, s->host[r].brick_x, s->host[r].brick_y);
+    {
+      const uint32_t gx = s->host[r].brick_x, gy = s->host[r].brick_y;
+      for (uint32_t z = 0; z < R.cells[2]; z++)
+        for (uint32_t y0 = 0; y0 < R.cells[1]; y0 += gy)
+          for (uint32_t x0 = 0; x0 < R.cells[0]; x0 += gx)
+            s->bricks.push_back({(uint32_t)r, (z * R.cells[1] + y0) * R.cells[0] + x0,
+                                 std::min(gx, R.cells[0] - x0), std::min(gy, R.cells[1] - y0)});
+    }
     for (uint32_t b = 0; b < R.count; b += kBlock)
       s->atom_blocks.push_back({(uint32_t)r, atom_off + b, atom_off + std::min(R.count, b + kBlock), 0});
     for (uint32_t b = 0; b < R.total_cells; b += kScanCells)
@@ -343,7 +394,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     atom_off += R.count;
     cell_off += R.total_cells;
     scan_off += R.scan_blocks;
-    list_off += (uint64_t)R.capacity * R.count;
+    list_off += (uint64_t)list_octets(R.capacity) * 8u * ((R.count + kListStride - 1) / kListStride) * kListStride;
     if (R.thermostat == 1) s->any_langevin = true;
   }
   s->n_total = atom_off;
@@ -373,11 +424,14 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   const size_t N = s->n_total, C = s->cells_total;
   Replica* rep_d = nullptr;
   Range *ab = nullptr, *sb = nullptr, *ob = nullptr;
+  Brick* gb = nullptr;
   A(&rep_d, replicas);
   A(&B.ctl, replicas);
   A(&ab, s->atom_blocks.size());
   A(&sb, s->scan_blocks.size());
   A(&ob, s->sort_blocks.size());
+  A(&gb, s->bricks.size());
+  A(&B.bmax, s->atom_blocks.size());
   double** d[] = {&B.x, &B.y, &B.z, &B.vx, &B.vy, &B.vz, &B.ax, &B.ay, &B.az,
                   &B.x2, &B.y2, &B.z2, &B.vx2, &B.vy2, &B.vz2, &B.ax2, &B.ay2, &B.az2,
                   &B.xb, &B.yb, &B.zb};
@@ -402,6 +456,8 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   B.atom_blocks = ab;
   B.scan_blocks = sb;
   B.sort_blocks = ob;
+  B.bricks = gb;
+  B.n_bricks = (uint32_t)s->bricks.size();
   B.n_atom_blocks = (uint32_t)s->atom_blocks.size();
   B.n_scan_blocks = (uint32_t)s->scan_blocks.size();
   B.n_sort_blocks = (uint32_t)s->sort_blocks.size();
@@ -432,6 +488,8 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   if (e == hipSuccess) e = hipMemcpy(ab, s->atom_blocks.data(), s->atom_blocks.size() * sizeof(Range), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(sb, s->scan_blocks.data(), s->scan_blocks.size() * sizeof(Range), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ob, s->sort_blocks.data(), s->sort_blocks.size() * sizeof(Range), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(gb, s->bricks.data(), s->bricks.size() * sizeof(Brick), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(B.bmax, 0, std::max<size_t>(s->atom_blocks.size(), 1) * 8);
   // Zeroed once here; afterwards the sort kernel leaves them zero.
   if (e == hipSuccess) e = hipMemset(B.cell_count, 0, std::max<size_t>(C, 1) * 4);
   if (e == hipSuccess) e = hipMemset(B.cell_cursor, 0, std::max<size_t>(C, 1) * 4);
@@ -648,7 +706,7 @@ int vlj_batch_read_particles(vlj_batch* sim, vlj_particle* out, size_t count) {
       const size_t slot = R.atom_off + t;
       vlj_particle& p = out[R.atom_off + orig[slot]];
       for (int k = 0; k < 3; k++) {
-        p.pos[k] = pos[3 * slot + k];
+        p.pos[k] = wrap_one(pos[3 * slot + k], R.box[k]);  // R1: wrapped on the way out
         p.vel[k] = vel[3 * slot + k];
       }
     }
@@ -822,12 +880,48 @@ int vlj_debug_lists(vlj_sim* sim, uint32_t* counts, uint32_t* entries) {
   if (rc != VLJ_OK) return rc;
   if (counts) std::memcpy(counts, cnt.data(), cnt.size() * 4);
   if (entries) {
-    std::vector<uint32_t> t((size_t)R.capacity * R.count);
-    rc = read_dev(sim, t.data(), B.nbr + R.list_off, t.size() * 4);
+    // Entries are staged indices of the atom's brick; rebuild each brick's
+    // staging order from the cell tables to turn them into slots.
+    const size_t padded = (size_t)((R.count + kListStride - 1) / kListStride) * kListStride;
+    std::vector<uint16_t> t((size_t)list_octets(R.capacity) * 8u * padded);
+    std::vector<uint32_t> cell_of(R.count), start(R.total_cells), ncell(R.total_cells), sorted(R.count);
+    rc = read_dev(sim, t.data(), B.nbr + R.list_off, t.size() * 2);
+    if (rc == VLJ_OK) rc = read_dev(sim, cell_of.data(), B.cell_of + R.atom_off, (size_t)R.count * 4);
+    if (rc == VLJ_OK) rc = read_dev(sim, start.data(), B.cell_start + R.cell_off, (size_t)R.total_cells * 4);
+    if (rc == VLJ_OK) rc = read_dev(sim, ncell.data(), B.cell_n + R.cell_off, (size_t)R.total_cells * 4);
+    if (rc == VLJ_OK) rc = read_dev(sim, sorted.data(), B.sorted + R.atom_off, (size_t)R.count * 4);
     if (rc != VLJ_OK) return rc;
-    for (uint32_t i = 0; i < R.count; i++)
-      for (uint32_t k = 0; k < R.capacity; k++)
-        entries[(size_t)i * R.capacity + k] = k < cnt[i] ? t[(size_t)k * R.count + i] - R.atom_off : 0u;
+    const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
+    const int bx = (int)sim->host[0].brick_x, by = (int)sim->host[0].brick_y;
+    for (uint32_t i = 0; i < R.count; i++) {
+      const int c = (int)(cell_of[i] - R.cell_off);
+      const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+      const int x0 = cx / bx * bx, y0 = cy / by * by;
+      const int gx = std::min(bx, nx - x0), gy = std::min(by, ny - y0);
+      std::vector<uint32_t> off, first;  // staged offset and first slot of each staged cell
+      uint32_t run = 0;
+      for (int dz = 0; dz < 3; dz++)
+        for (int yy = 0; yy < gy + 2; yy++)
+          for (int xx = 0; xx < gx + 2; xx++) {
+            const int ccx = (x0 - 1 + xx + nx) % nx, ccy = (y0 - 1 + yy + ny) % ny, ccz = (cz - 1 + dz + nz) % nz;
+            const int cell = (ccz * ny + ccy) * nx + ccx;
+            off.push_back(run);
+            first.push_back(start[cell] - R.atom_off);
+            run += ncell[cell];
+          }
+      off.push_back(run);
+      const uint64_t base = list_base(R, i) - R.list_off;
+      for (uint32_t k = 0; k < R.capacity; k++) {
+        uint32_t slot = 0;
+        if (k < cnt[i]) {
+          const uint32_t e = t[base + list_entry(k)];
+          size_t sc = std::upper_bound(off.begin(), off.end(), e) - off.begin() - 1;
+          while (sc + 1 < off.size() && off[sc + 1] <= e) sc++;
+          slot = sorted[first[sc] + (e - off[sc])] - R.atom_off;
+        }
+        entries[(size_t)i * R.capacity + k] = slot;
+      }
+    }
   }
   return VLJ_OK;
 }
@@ -856,6 +950,8 @@ int vlj_debug_params(const vlj_sim* sim, double out[6]) {
   out[5] = sim->rep[0].c2;
   return VLJ_OK;
 }
+
+uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks) { return xcd_swizzle(block, blocks); }
 
 int vlj_timed_run(vlj_sim* sim, uint64_t steps, double* force_ms, double* total_ms) {
   if (!sim || !force_ms || !total_ms) return fail(sim, VLJ_E_INVALID, "bad arguments to vlj_timed_run");

@@ -114,6 +114,7 @@ SIGNATURES = {
     "vlj_debug_lists": (C.c_int, [_P, _up, _up]),
     "vlj_debug_grid": (C.c_int, [_P, _up]),
     "vlj_debug_params": (C.c_int, [_P, _dp]),
+    "vlj_debug_swizzle": (_u32, [_u32, _u32]),
     "vlj_timed_run": (C.c_int, [_P, _u64, _dp, _dp]),
 }
 
