@@ -99,8 +99,8 @@ typedef struct vlj_sim vlj_batch;
 /* LennardJones::new (lennard_jones.rs:443-801): copies the particles
  * (positions wrapped into the box), rounds the box to f32, derives cells,
  * list capacity, fixed-point scales and pair constants. VLJ_E_INVALID with
- * "the box must be at least three (cutoff + skin) wide on every axis" where
- * the reference asserts (rs:458-461). No kernel runs until the first step or
+ * "the box must be at least two (cutoff + skin) wide on every axis" where
+ * the reference asserts three (rs:458-461). No kernel runs until the first step or
  * measurement (lazy priming, rs:1080-1089). */
 int vlj_create(const vlj_particle *particles, size_t count, const vlj_settings *settings,
                vlj_sim **out);

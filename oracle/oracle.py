@@ -212,7 +212,7 @@ def derive_params(pos, vel, box, cutoff=2.5, skin=0.3, dt=0.005, shifted_force=F
     rc = lib().vo_derive_params(_dp(pos), pos.shape[0], _dp(b), cutoff, skin, dt, int(shifted_force), precision,
                                 thermostat, temperature, friction, seed, _dp(vel), C.byref(p))
     if rc != 0:
-        raise ValueError("the box must be at least three (cutoff + skin) wide on every axis")
+        raise ValueError("the box must be at least two (cutoff + skin) wide on every axis")
     return p
 
 
@@ -260,7 +260,7 @@ class Sim:
         self._h = lib().vo_create(_dp(pos), _dp(vel), self.n, _dp(b), cutoff, skin, dt, int(shifted_force),
                                   precision, thermostat, temperature, friction, seed)
         if not self._h:
-            raise ValueError("the box must be at least three (cutoff + skin) wide on every axis")
+            raise ValueError("the box must be at least two (cutoff + skin) wide on every axis")
         self.params = Params()
         lib().vo_get_params(self._h, C.byref(self.params))
 

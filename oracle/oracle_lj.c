@@ -22,7 +22,8 @@
  *     (lennard_jones.wgsl:454-499). The displacement is taken to the image
  *     of j's cell seen from i's cell, d = (xj + s L) - xi with s = -1 where
  *     c + d wraps below 0, +1 where it wraps past n - 1, else 0; for n >= 3
- *     this is the nearest image of every pair within one cell width
+ *     (n >= 2 is enough: see vo_derive_params) this is the nearest image of
+ *     every pair within one cell width
  *     (the reference's nearest(), lennard_jones.wgsl:198-202). Each entry
  *     keeps its image s.
  *  R5 pair, FP64, over the list in order with d = (xj + s L) - xi and the
@@ -425,14 +426,15 @@ void vo_sincos_turns(double u, double *s_out, double *c_out)
 }
 
 /* Three standard normals for one atom and step: Philox4x32-10 with counter
- * (step low, step high, original atom index, replica) and key (seed low,
+ * (step low, step high, original atom index, stream) and key (seed low,
  * seed high); uniforms (w + 0.5) 2^-32; Box-Muller on (u0, u1) gives two,
  * the cosine branch on (u2, u3) the third. The reference draws four and uses
- * three per step as well (lennard_jones.wgsl:242-247), from a stateful PCG. */
-void vo_langevin_noise(uint64_t seed, uint32_t replica, uint32_t atom, uint64_t step,
+ * three per step as well (lennard_jones.wgsl:242-247), from a stateful PCG.
+ * The thermostat uses stream 0: replicas differ by seed only. */
+void vo_langevin_noise(uint64_t seed, uint32_t stream, uint32_t atom, uint64_t step,
                        double out[3])
 {
-    uint32_t ctr[4] = {(uint32_t)step, (uint32_t)(step >> 32), atom, replica};
+    uint32_t ctr[4] = {(uint32_t)step, (uint32_t)(step >> 32), atom, stream};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t w[4];
     vo_philox4x32(ctr, key, w);
@@ -494,8 +496,12 @@ int vo_derive_params(const double *pos, size_t n, const double box_in[3], double
         double c = floor(p->box[k] / list_radius);
         p->cells[k] = c < 0 ? 0 : (uint32_t)c;
     }
+    /* The reference asks for 3 cells per axis (lennard_jones.rs:458-461);
+     * here 2 suffice: with L >= 2 (rc + skin) at most one image of a pair
+     * is within rc + skin, so a stencil that meets a cell twice (once per
+     * image) still lists each pair once. */
     for (int k = 0; k < 3; k++)
-        if (p->cells[k] < 3)
+        if (p->cells[k] < 2)
             return -1;
     while ((uint64_t)p->cells[0] * p->cells[1] * p->cells[2] > (1u << 20)) {
         int longest = 0;

@@ -29,7 +29,7 @@ extern "C" {
 typedef struct {
     double box[3];          /* requested sides rounded to f32 (rs:450) */
     double half_box[3];
-    uint32_t cells[3];      /* floor(L / (rc + skin)), >= 3 (rs:457-465) */
+    uint32_t cells[3];      /* floor(L / (rc + skin)), >= 2 (rs:457-465 asks 3) */
     uint32_t total_cells;
     float cell_size[3];     /* f32 cell sides used by the binning rule (rs:533) */
     uint32_t count;
@@ -110,7 +110,7 @@ void vo_all_pairs(const double *pos, size_t n, const double box[3], double cutof
 
 /* Counter-based random numbers for the Langevin noise. */
 void vo_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
-void vo_langevin_noise(uint64_t seed, uint32_t replica, uint32_t atom, uint64_t step,
+void vo_langevin_noise(uint64_t seed, uint32_t stream, uint32_t atom, uint64_t step,
                        double out[3]);
 double vo_log(double x);
 void vo_sincos_turns(double u, double *s, double *c);

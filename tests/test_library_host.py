@@ -31,8 +31,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_create_validates_before_touching_a_device(lj):
     pos = np.zeros((10, 3))
-    with pytest.raises(lj.LennardJonesError, match="at least three \\(cutoff \\+ skin\\) wide") as e:
-        lj.LennardJones((pos, pos), lj.Settings([8.0, 20.0, 20.0], 2.5, 0.005))
+    with pytest.raises(lj.LennardJonesError, match="at least two \\(cutoff \\+ skin\\) wide") as e:
+        lj.LennardJones((pos, pos), lj.Settings([5.5, 20.0, 20.0], 2.5, 0.005))
     assert e.value.code == 1
     with pytest.raises(lj.LennardJonesError, match="Nose-Hoover") as e:
         lj.LennardJones((pos, pos), lj.Settings([20.0] * 3, 2.5, 0.005, thermostat=lj.NoseHoover(1.0, 0.5)))

@@ -129,6 +129,8 @@ def test_energy_scale_of_the_million_atom_liquid(oracle):
     assert p.capacity >= p.capacity_reference
 
 
-def test_box_narrower_than_three_cells_is_refused(oracle):
-    with pytest.raises(ValueError, match="three"):
-        oracle.derive_params(np.zeros((1, 3)), np.zeros((1, 3)), [8.0, 20.0, 20.0])
+def test_box_narrower_than_two_list_radii_is_refused(oracle):
+    with pytest.raises(ValueError, match="two"):
+        oracle.derive_params(np.zeros((1, 3)), np.zeros((1, 3)), [5.5, 20.0, 20.0])
+    p = oracle.derive_params(np.zeros((1, 3)), np.zeros((1, 3)), [5.61, 20.0, 20.0])
+    assert p.cells[0] == 2

@@ -239,12 +239,14 @@ __device__ __forceinline__ void sincos_turns(double u, double* s_out, double* c_
   *c_out = co;
 }
 
-// Three standard normals for (seed, replica, original atom, step): counter
-// (step lo, step hi, atom, replica), key (seed lo, seed hi); uniforms
+// Three standard normals for (seed, stream, original atom, step): counter
+// (step lo, step hi, atom, stream), key (seed lo, seed hi); uniforms
 // (w + 0.5) 2^-32; Box-Muller on (u0, u1) and the cosine branch on (u2, u3).
-__device__ __forceinline__ void langevin_noise(uint64_t seed, uint32_t replica, uint32_t atom,
+// The thermostat uses stream 0: replicas are told apart by their seeds, so a
+// replica's noise does not depend on its place in a batch.
+__device__ __forceinline__ void langevin_noise(uint64_t seed, uint32_t stream, uint32_t atom,
                                                unsigned long long step, double n[3]) {
-  uint32_t c[4] = {(uint32_t)step, (uint32_t)(step >> 32), atom, replica};
+  uint32_t c[4] = {(uint32_t)step, (uint32_t)(step >> 32), atom, stream};
   philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
   double u0 = ((double)c[0] + 0.5) * 2.3283064365386963e-10;
   double u1 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;

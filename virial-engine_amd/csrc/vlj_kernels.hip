@@ -1,16 +1,16 @@
 // HIP kernels of the Lennard-Jones path, written for CDNA4 (gfx950, wave64).
 //
 // One step is (lennard_jones.rs:1092-1138):
-//   drift       half kick + drift (or BAOAB), wrap, max displacement; the last
-//               workgroup to finish takes the rebuild decision for every
-//               replica (lennard_jones.wgsl:204-307)
-//   bin         cell of each atom, count per cell          } rebuild chain,
-//   scan        exclusive prefix sum of counts             } each kernel
-//   scatter     atoms into their cell's range              } returns at once
-//   sort        each cell's slots ascending (wave per cell)} when its replica
-//   reorder     atoms into cell order (every 4th rebuild)  } does not rebuild
-//   build       Verlet lists, transposed                   } (grid.wgsl, wgsl:419-499)
-//   force       pair forces from the lists + second half kick (wgsl:501-574)
+//   drift       half kick + drift (or BAOAB), displacement since the last
+//               build, one maximum per workgroup (lennard_jones.wgsl:204-276)
+//   decide      per replica: rebuild or not (lennard_jones.wgsl:278-307)
+//   bin         wrap into the box, cell of each atom, count   } rebuild chain,
+//   scan        exclusive prefix sum of counts                } each kernel
+//   scatter     atoms into their cell's range                 } returns at once
+//   sort        each cell's slots ascending (wave per cell)   } when its replica
+//   reorder     atoms into cell order                         } does not rebuild
+//   build       brick-staged Verlet lists                     } (grid.wgsl, wgsl:419-499)
+//   force       brick-staged pair forces + second half kick (wgsl:501-574)
 // The sequence is static, so it is captured once into a hipGraph.
 #include "vlj_device.h"
 #include "vlj_launch.h"
@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
   if (i < blk.end) {
     const bool baoab = LANGEVIN && R.thermostat == 1u;
     double n[3] = {0.0, 0.0, 0.0};
-    if (baoab) langevin_noise(R.seed, R.index, B.orig[i], B.ctl[blk.rep].step, n);
+    if (baoab) langevin_noise(R.seed, 0u, B.orig[i], B.ctl[blk.rep].step, n);
     double* X[3] = {B.x, B.y, B.z};
     double* V[3] = {B.vx, B.vy, B.vz};
     const double* A[3] = {B.ax, B.ay, B.az};
@@ -567,11 +567,12 @@ __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
             }
           }
       list.finish();
-    } else {
-      list.found = total;  // staged indices would not fit in 16 bits
     }
-    const uint32_t found = list.found;
-    B.nbr_count[i] = found < cap ? found : 0u;
+    // Past the capacity the list keeps its first `cap` entries and the
+    // overflow is reported (lennard_jones.wgsl:494-497); a neighbourhood too
+    // large for 16-bit staged indices keeps none.
+    const uint32_t found = total <= 0xffffu ? list.found : total;
+    B.nbr_count[i] = total <= 0xffffu ? (found < cap ? found : cap) : 0u;
     if (found > cap) atomicMax(&B.ctl[g.rep].overflow, found);
     B.xb[i] = xi;
     B.yb[i] = yi;
