@@ -68,7 +68,7 @@ typedef struct {
     double skin;
     vlj_thermostat thermostat;
     int32_t shifted_force;
-    uint32_t profile_bins;
+    uint32_t profile_bins;  /* slabs along z for the profiles (0: none), at most 2^20 */
     uint32_t precision;
 } vlj_settings;
 
@@ -137,14 +137,26 @@ int vlj_set_temperature(vlj_sim *sim, double temperature);
 /* set_condensed_neighbours (rs:895-899). */
 int vlj_set_condensed_neighbours(vlj_sim *sim, uint32_t neighbours);
 
-/* Profiles along z (rs:1167-1191, 1297-1325): sums[bins * 8 + 3] are the
- * running fixed-point sums decoded to doubles (per slab: count, m v_a^2 for
- * a = x, y, z, W_aa, potential; then the whole-box W_aa), not yet divided by
- * the sample count. VLJ_E_STATE "profiles need Settings::profile_bins above
- * zero" without bins. */
+/* Profiles along z (lennard_jones.wgsl:628-730). Layout of the
+ * profile_bins * 8 + 3 values: per slab b from b * 8 the particle count,
+ * m v_a^2 for a = x, y, z, the Irving-Kirkwood virial diagonal W_aa and the
+ * potential energy; then the whole-box W_aa summed pair by pair.
+ *
+ * vlj_sample_profile (rs:1167-1183) adds one sample to the running sums of
+ * every replica with profile slabs, asynchronously; VLJ_E_STATE "profiles
+ * need Settings::profile_bins above zero" when none has any.
+ * vlj_clear_profile (rs:1186-1191) starts the sums afresh.
+ * vlj_read_profile (rs:1297-1325) writes the sums divided by the sample
+ * count (NaN before the first sample), as the reference's Profile holds
+ * them; capacity must be at least profile_bins * 8 + 3.
+ * vlj_read_profile_raw writes the 64-bit fixed-point sums (value * 2^32)
+ * themselves, and vlj_batch_read_profile_raw those of one replica. */
 int vlj_sample_profile(vlj_sim *sim);
 int vlj_clear_profile(vlj_sim *sim);
-int vlj_read_profile(vlj_sim *sim, double *sums, size_t capacity, uint64_t *samples);
+int vlj_read_profile(vlj_sim *sim, double *out, size_t capacity, uint64_t *samples);
+int vlj_read_profile_raw(vlj_sim *sim, int64_t *raw, size_t capacity, uint64_t *samples);
+int vlj_batch_read_profile_raw(vlj_batch *batch, size_t replica, int64_t *raw, size_t capacity,
+                               uint64_t *samples);
 
 /* Last error message of the handle (or of the last failed vlj_create when
  * sim is NULL). Never NULL. */
@@ -202,6 +214,12 @@ int vlj_debug_params(const vlj_sim *sim, double out[6]);
  * on: workgroups go round-robin to the 8 XCDs, and each XCD gets one
  * contiguous share of the table. A bijection on [0, blocks). */
 uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks);
+
+/* Plain streaming copy of `bytes` (read once, written once) between two
+ * device buffers, `repeats` times; *ms = average kernel time from HIP
+ * events. Measures the bandwidth a streaming kernel reaches and calibrates
+ * rocprof byte counters (no counterpart in the reference). */
+int vlj_copy_bandwidth(size_t bytes, uint32_t repeats, double *ms);
 
 /* Kernel timing with HIP events on the handle's stream: runs `steps` steps
  * launching the kernels directly (not from the captured graph) with an event

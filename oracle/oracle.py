@@ -115,6 +115,10 @@ def lib() -> C.CDLL:
         L.vo_read_energies.argtypes = [P, dp, dp]
         L.vo_rebuilds.argtypes = [P]
         L.vo_rebuilds.restype = u32
+        L.vo_set_profile_bins.argtypes = [P, u32]
+        L.vo_clear_profile.argtypes = [P]
+        L.vo_sample_profile.argtypes = [P]
+        L.vo_read_profile.argtypes = [P, C.POINTER(i64), C.POINTER(u64)]
         L.vo_all_pairs.argtypes = [dp, sz, dp, d, i32, dp, dp, dp]
         L.vo_philox4x32.argtypes = [up, up, up]
         L.vo_langevin_noise.argtypes = [u64, u32, u32, u64, dp]
@@ -321,3 +325,21 @@ class Sim:
 
     def rebuilds(self) -> int:
         return lib().vo_rebuilds(self._h)
+
+    # Profiles along z (lennard_jones.wgsl:628-730): raw fixed-point sums,
+    # bins * 8 + 3 of them, and the sample count.
+    def set_profile_bins(self, bins: int):
+        self.profile_bins = int(bins)
+        lib().vo_set_profile_bins(self._h, self.profile_bins)
+
+    def clear_profile(self):
+        lib().vo_clear_profile(self._h)
+
+    def sample_profile(self):
+        lib().vo_sample_profile(self._h)
+
+    def read_profile(self):
+        raw = np.zeros(self.profile_bins * 8 + 3, dtype=np.int64)
+        samples = C.c_uint64(0)
+        lib().vo_read_profile(self._h, raw.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(samples))
+        return raw, samples.value

@@ -70,6 +70,9 @@ struct vo_sim {
     uint32_t *near, *within;
     uint32_t rebuilds, overflow;
     int built, primed;
+    uint32_t prof_bins;
+    uint64_t prof_samples;
+    int64_t *prof;
     double max_moved2;
     uint64_t steps;
 };
@@ -651,6 +654,7 @@ void vo_destroy(vo_sim *s)
     free(s->w);
     free(s->near);
     free(s->within);
+    free(s->prof);
     free(s);
 }
 
@@ -941,6 +945,133 @@ void vo_run(vo_sim *s, uint64_t steps)
         compute_forces(s, 1);
         s->steps++;
     }
+}
+
+/* Pair terms r.F and U at r2 < rc2, in FP64 whatever the force precision
+ * (the R5 sequence). */
+static void pair_terms(const vo_params *p, double r2, double *rf_out, double *u_out)
+{
+    double inv2 = 1.0 / r2;
+    double s6 = inv2 * inv2 * inv2;
+    double rf = s6 * fma(48.0, s6, -24.0);
+    double t = 4.0 * s6;
+    double u = fma(t, s6, -t) - p->shift;
+    if (p->shifted_force) {
+        double r = sqrt(r2);
+        rf = fma(-r, p->force_shift, rf);
+        u = fma(r - p->rc, p->force_shift, u);
+    }
+    *rf_out = rf;
+    *u_out = u;
+}
+
+/* R10 profiles along z (lennard_jones.wgsl:628-730): per slab the count,
+ * m v_a^2, the Irving-Kirkwood virial diagonal W_aa and the potential, then
+ * the whole-box W_aa. Each (pair, slab, axis) contribution and each per-atom
+ * value goes to fixed point separately, llrint(x 2^32), as the reference
+ * adds each with fixed_add; the integers are summed. A pair's half of the
+ * virial, w_a = (0.5 d_a) d_a (rf / r2), is spread over the slabs the
+ * segment from z_i to z_i + d_z crosses, in proportion to the length inside
+ * each, the last slab taking what is left. Slabs use the wrapped z. The
+ * pair terms are FP64 in both precision forms. The walk over slabs stops
+ * after bins + 2 slabs, more than a segment shorter than the box can cross. */
+static inline void prof_add(int64_t *prof, uint32_t slab, int q, double x)
+{
+    prof[slab * 8u + (uint32_t)q] += llrint(x * TWO32);
+}
+
+void vo_set_profile_bins(vo_sim *s, uint32_t bins)
+{
+    free(s->prof);
+    s->prof_bins = bins;
+    s->prof = bins ? (int64_t *)xcalloc((size_t)bins * 8 + 3, sizeof(int64_t)) : NULL;
+    s->prof_samples = 0;
+}
+
+void vo_clear_profile(vo_sim *s)
+{
+    if (s->prof)
+        memset(s->prof, 0, ((size_t)s->prof_bins * 8 + 3) * sizeof(int64_t));
+    s->prof_samples = 0;
+}
+
+void vo_sample_profile(vo_sim *s)
+{
+    vo_prime(s);
+    const vo_params *p = &s->p;
+    const uint32_t bins = s->prof_bins;
+    if (!bins)
+        return;
+    int64_t *prof = s->prof;
+    const double width = p->box[2] / (double)bins;
+    for (uint32_t i = 0; i < p->count; i++) {
+        const double xi = s->x[0][i], yi = s->x[1][i], zraw = s->x[2][i];
+        const double zi = wrap_step(zraw, p->box[2]);
+        int hb = (int)floor(zi / width);
+        const uint32_t home = (uint32_t)(hb < 0 ? 0 : (hb > (int)bins - 1 ? (int)bins - 1 : hb));
+        prof_add(prof, home, 0, 1.0);
+        for (int k = 0; k < 3; k++)
+            prof_add(prof, home, 1 + k, s->v[k][i] * s->v[k][i]);
+        double pe = 0.0, total[3] = {0.0, 0.0, 0.0};
+        const uint32_t cnt = s->nbr_count[i];
+        const uint32_t *row = s->nbr + (size_t)i * p->capacity;
+        const uint8_t *img = s->nbr_image + (size_t)i * p->capacity;
+        for (uint32_t k = 0; k < cnt; k++) {
+            const uint32_t j = row[k];
+            const int code = img[k];
+            const double d[3] = {(s->x[0][j] + (code % 3 - 1) * p->box[0]) - xi,
+                                 (s->x[1][j] + ((code / 3) % 3 - 1) * p->box[1]) - yi,
+                                 (s->x[2][j] + (code / 9 - 1) * p->box[2]) - zraw};
+            const double r2 = fma(d[2], d[2], fma(d[1], d[1], d[0] * d[0]));
+            if (!(r2 < p->rc2))
+                continue;
+            double rf, u;
+            pair_terms(p, r2, &rf, &u);
+            pe += 0.5 * u;
+            const double q = rf / r2;
+            double w[3];
+            for (int a = 0; a < 3; a++) {
+                w[a] = (0.5 * d[a]) * d[a] * q;
+                total[a] += w[a];
+            }
+            const double lo = zi + (d[2] < 0.0 ? d[2] : 0.0);
+            const double hi = zi + (d[2] > 0.0 ? d[2] : 0.0);
+            const double length = hi - lo;
+            if (length < 1e-6 * width) {
+                for (int a = 0; a < 3; a++)
+                    prof_add(prof, home, 4 + a, w[a]);
+                continue;
+            }
+            int b = (int)floor(lo / width);
+            double left = 1.0;
+            for (uint32_t walk = 0; walk < bins + 2u; walk++) {
+                const int last = (double)(b + 1) * width >= hi;
+                const double start = lo > (double)b * width ? lo : (double)b * width;
+                const double end = hi < (double)(b + 1) * width ? hi : (double)(b + 1) * width;
+                double share = last ? left : (end - start) / length;
+                share = share > 0.0 ? share : 0.0;
+                left -= share;
+                const uint32_t slab = (uint32_t)(((b % (int)bins) + (int)bins) % (int)bins);
+                for (int a = 0; a < 3; a++)
+                    prof_add(prof, slab, 4 + a, w[a] * share);
+                if (last)
+                    break;
+                b++;
+            }
+        }
+        prof_add(prof, home, 7, pe);
+        for (int a = 0; a < 3; a++)
+            prof[(size_t)bins * 8u + (uint32_t)a] += llrint(total[a] * TWO32);
+    }
+    s->prof_samples++;
+}
+
+void vo_read_profile(const vo_sim *s, int64_t *raw, uint64_t *samples)
+{
+    if (raw && s->prof)
+        memcpy(raw, s->prof, ((size_t)s->prof_bins * 8 + 3) * sizeof(int64_t));
+    if (samples)
+        *samples = s->prof_samples;
 }
 
 /* R9, lennard_jones.wgsl:576-626 and lennard_jones.rs:1209-1246 */

@@ -102,6 +102,11 @@ void vo_read_cells(const vo_sim *s, uint32_t *cell_of, uint32_t *cell_start, uin
 void vo_read_lists(const vo_sim *s, uint32_t *counts, uint32_t *entries);
 void vo_read_energies(const vo_sim *s, double *pe, double *w);
 uint32_t vo_rebuilds(const vo_sim *s);
+/* Profiles along z (lennard_jones.wgsl:628-730): bins * 8 + 3 int64 sums. */
+void vo_set_profile_bins(vo_sim *s, uint32_t bins);
+void vo_clear_profile(vo_sim *s);
+void vo_sample_profile(vo_sim *s);
+void vo_read_profile(const vo_sim *s, int64_t *raw, uint64_t *samples);
 
 /* Brute-force all pairs, nearest image (tests/thermodynamics.rs:20-53,
  * tests/coexistence.rs:38-76). Any output pointer may be NULL. */

@@ -20,6 +20,8 @@ namespace vlj {
 constexpr int kBlock = 256;       // threads per atom block (4 wave64)
 constexpr int kScanCells = 1024;  // cells per scan block (256 threads x 4)
 constexpr int kSortCells = 4;     // cells per sort block (one wave64 per cell)
+constexpr uint32_t kProfileSums = 8;       // sums per profile slab (lennard_jones.wgsl:644)
+constexpr uint32_t kMaxProfileBins = 1u << 20;  // profile slabs per replica, at most
 constexpr int kBrickMax = 6;      // home cells of a brick along x and along y, at most
 constexpr int kStageCells = 3 * (kBrickMax + 2) * (kBrickMax + 2);  // staged cells, at most
 constexpr int kHomeCells = kBrickMax * kBrickMax;                    // home cells, at most
@@ -49,6 +51,9 @@ struct Replica {
   uint64_t seed;
   uint32_t near_count;
   uint32_t index;
+  double bin_width;    // profile slab height, L_z / prof_bins
+  uint32_t prof_bins;  // profile slabs along z (0: no profiles)
+  uint32_t prof_off;   // first of the replica's prof_bins * 8 + 3 profile sums
 };
 
 // Control words of one replica (lennard_jones.wgsl:106-110: MAX_DISPLACEMENT,
@@ -96,6 +101,7 @@ struct Buffers {
   uint32_t* nbr_count;                                       // list length of each slot
   uint16_t* nbr;                                             // lists of staged indices
   long long* slots;                                          // kSlots per replica
+  long long* profile;                                        // profile sums of all replicas
   uint32_t* tickets;                                         // last-block counters
 };
 
@@ -114,8 +120,13 @@ __host__ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t n)
 // is at (u16 units)
 //   list_off + (li / 64) * octets * 512 + o * 512 + (li % 64) * 8,
 // octets = ceil(capacity / 8). A wave reads octet o of its 64 atoms as 1 KB
-// of contiguous memory; all of a wave's lists lie in octets KB.
+// of contiguous memory; all of a wave's lists lie in octets KB. The last
+// octet of a list is filled up with kListPad, the staged index of a sentinel
+// position far outside every box, so the force loop can take whole octets
+// without a per-entry bound.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kListPad = (uint32_t)kStageMax;
+constexpr double kFar = 1.0e18;  // sentinel coordinate: r^2 ~ 3e36, beyond any cutoff
 
 __host__ __device__ __forceinline__ uint32_t list_octets(uint32_t capacity) { return (capacity + 7u) / 8u; }
 

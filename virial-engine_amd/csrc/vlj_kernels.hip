@@ -316,7 +316,9 @@ struct Stage {
   uint8_t image[kStageCells];     // (ix + 1) + 3 (iy + 1) + 9 (iz + 1), i in {-1, 0, 1}
   uint32_t ns;                    // staged cells
   uint32_t total;                 // staged atoms
-  double x[kStageMax], y[kStageMax], z[kStageMax];
+  // Staged positions and, at kListPad, the sentinel. The arrays are 2112
+  // long (33 x 64 doubles) so that x[e] and y[e] are one ds_read2st64_b64.
+  double x[kStageMax + 64], y[kStageMax + 64], z[kStageMax + 64];
 };
 
 // Staged index of the cell at offset (dz, yy, xx) from the brick's corner
@@ -422,6 +424,7 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
         S.z[e] = pz[k] + shz;
       }
     }
+    if (t == 0) S.x[kListPad] = S.y[kListPad] = S.z[kListPad] = kFar;
   }
   __syncthreads();
   return staged;
@@ -470,26 +473,33 @@ __device__ __forceinline__ void staged_pos(const Replica& R, const Stage& S, boo
 // ---------------------------------------------------------------------------
 
 // Appends one atom's list entries eight at a time (one 16-byte store).
+// The octet being filled is a 128-bit shift register (lo, hi): each entry
+// comes in at the top and moves the others down by 16 bits, so after eight
+// entries the first is in the low bits.
 struct ListWriter {
   uint16_t* base;  // list_base of the atom
-  u32x4 buf;
+  unsigned long long lo, hi;
   uint32_t found, cap;
+  __device__ __forceinline__ void store(uint32_t first) {
+    *(u32x4*)(base + list_entry(first)) = u32x4{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
+  }
   __device__ __forceinline__ void push(uint32_t e) {
     if (found < cap) {
-      const uint32_t s = found & 7u, w = s >> 1;
-      const uint32_t word = (w == 0u ? buf.x : (w == 1u ? buf.y : (w == 2u ? buf.z : buf.w)));
-      const uint32_t merged = (s & 1u) ? ((word & 0xffffu) | (e << 16)) : ((word & 0xffff0000u) | e);
-      buf.x = w == 0u ? merged : buf.x;
-      buf.y = w == 1u ? merged : buf.y;
-      buf.z = w == 2u ? merged : buf.z;
-      buf.w = w == 3u ? merged : buf.w;
-      if (s == 7u) *(u32x4*)(base + list_entry(found & ~7u)) = buf;
+      lo = (lo >> 16) | (hi << 48);
+      hi = (hi >> 16) | ((unsigned long long)e << 48);
+      if ((found & 7u) == 7u) store(found & ~7u);
     }
     found++;
   }
   __device__ __forceinline__ void finish() {
     const uint32_t stored = found < cap ? found : cap;
-    if (stored & 7u) *(u32x4*)(base + list_entry(stored & ~7u)) = buf;
+    if (stored & 7u) {
+      for (uint32_t k = stored & 7u; k < 8u; k++) {  // fill the partial octet with pads
+        lo = (lo >> 16) | (hi << 48);
+        hi = (hi >> 16) | ((unsigned long long)kListPad << 48);
+      }
+      store(stored & ~7u);
+    }
   }
 };
 
@@ -529,7 +539,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
       B.orig[i] = B.orig2[i];
       B.cell_of[i] = B.cell_of2[i];
     }
-    ListWriter list = {B.nbr + list_base(R, i - R.atom_off), u32x4{0u, 0u, 0u, 0u}, 0u, cap};
+    ListWriter list = {B.nbr + list_base(R, i - R.atom_off), 0ull, 0ull, 0u, cap};
     if (total <= 0xffffu) {
       for (uint32_t dz = 0; dz < 3u; dz++)
         for (uint32_t dy = 0; dy < 3u; dy++)
@@ -537,21 +547,24 @@ __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
             const uint32_t sc = staged_cell(g, dz, a.hy + dy, a.hx + dx);
             const uint32_t e0 = S.off[sc], e1 = S.off[sc + 1u];
             if (staged) {
-              // four candidates per pass: all LDS reads first, then the
-              // appends in order
-              for (uint32_t e2 = e0; e2 < e1; e2 += 4u) {
-                bool in[4];
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                  const uint32_t q = e2 + (uint32_t)c;
-                  const uint32_t qq = q < e1 ? q : e0;
-                  const double ddx = S.x[qq] - xi, ddy = S.y[qq] - yi, ddz = S.z[qq] - zi;
+              // 64 candidates at a time: a branch-free pass sets one bit per
+              // atom within rc + skin, then only the set bits are appended,
+              // lowest first (slot order)
+              for (uint32_t c0 = e0; c0 < e1; c0 += 64u) {
+                const uint32_t m = e1 - c0 < 64u ? e1 - c0 : 64u;
+                unsigned long long in = 0ull;
+                for (uint32_t c = 0; c < m; c++) {
+                  const uint32_t q = c0 + c;
+                  const double ddx = S.x[q] - xi, ddy = S.y[q] - yi, ddz = S.z[q] - zi;
                   // q == a.e is the atom itself (its home copy, unshifted)
-                  in[c] = q < e1 && q != a.e && dist2(ddx, ddy, ddz) < R.list_r2;
+                  const bool hit = q != a.e && dist2(ddx, ddy, ddz) < R.list_r2;
+                  in |= (unsigned long long)hit << c;
                 }
-#pragma unroll
-                for (int c = 0; c < 4; c++)
-                  if (in[c]) list.push(e2 + (uint32_t)c);
+                while (in) {
+                  const uint32_t c = (uint32_t)__builtin_ctzll(in);
+                  in &= in - 1ull;
+                  list.push(c0 + c);
+                }
               }
             } else {
               double shx, shy, shz;
@@ -590,10 +603,14 @@ struct PairAcc {
   uint32_t near, within;
 };
 
-// R5 (FP64) and R5f (FP32 pair arithmetic on FP64 displacements).
+// R5 (FP64) and R5f (FP32 pair arithmetic on FP64 displacements), without
+// branches: the terms of a pair outside the cutoff are computed and then
+// replaced by zero, which leaves every sum bit for bit as if the pair had
+// been skipped (no sum here is ever -0, so adding +0 changes nothing).
 template <int PREC, bool SF, bool ENERGIES>
 __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, double dy, double dz,
                                                 double r2, PairAcc& a) {
+  const bool hit = r2 < R.rc2;
   if (PREC == 0) {
     const double inv2 = 1.0 / r2;
     const double s6 = inv2 * inv2 * inv2;
@@ -608,13 +625,13 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
       rf = __builtin_fma(-r, R.force_shift, rf);
       if (ENERGIES) u = __builtin_fma(r - R.rc, R.force_shift, u);
     }
-    const double fscal = rf * inv2;
+    const double fscal = hit ? rf * inv2 : 0.0;
     a.ax = __builtin_fma(-dx, fscal, a.ax);
     a.ay = __builtin_fma(-dy, fscal, a.ay);
     a.az = __builtin_fma(-dz, fscal, a.az);
     if (ENERGIES) {
-      a.pe += 0.5 * u;
-      a.w += 0.5 * rf;
+      a.pe += hit ? 0.5 * u : 0.0;
+      a.w += hit ? 0.5 * rf : 0.0;
     }
   } else {
     const float q = (float)r2;
@@ -632,18 +649,18 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
       rf = __builtin_fmaf(-r, fs, rf);
       if (ENERGIES) u = __builtin_fmaf(r - (float)R.rc, fs, u);
     }
-    const float fscal = rf * inv2;
+    const float fscal = hit ? rf * inv2 : 0.0f;
     a.ax -= (double)((float)dx * fscal);
     a.ay -= (double)((float)dy * fscal);
     a.az -= (double)((float)dz * fscal);
     if (ENERGIES) {
-      a.pe += 0.5 * (double)u;
-      a.w += 0.5 * (double)rf;
+      a.pe += hit ? 0.5 * (double)u : 0.0;
+      a.w += hit ? 0.5 * (double)rf : 0.0;
     }
   }
   if (ENERGIES) {
-    a.near += r2 < kNearR2 ? 1u : 0u;
-    a.within += 1u;
+    a.near += hit && r2 < kNearR2 ? 1u : 0u;
+    a.within += hit ? 1u : 0u;
   }
 }
 
@@ -656,10 +673,12 @@ __device__ __forceinline__ u32x4 load_octet(const uint16_t* base, uint32_t o, ui
 
 // Atom i's list in order, kOctetBatch octets at a time, double-buffered:
 // the next batch's loads are in flight while this one is summed. Positions
-// come from the brick's LDS.
+// come from the brick's LDS; whole octets are taken (pads meet the far
+// sentinel), so the eight pairs of an octet carry no branch and their LDS
+// reads can all be in flight together.
 template <int PREC, bool SF, bool ENERGIES>
-__device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica& R, const Stage& S, bool staged,
-                                                uint32_t i, double xi, double yi, double zi) {
+__device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const Replica& R, const Stage& S,
+                                                       uint32_t i, double xi, double yi, double zi) {
   const uint32_t cnt = B.nbr_count[i];
   const uint32_t no = (cnt + 7u) / 8u;
   const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
@@ -672,23 +691,49 @@ __device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica&
     for (int b = 0; b < kOctetBatch; b++) nxt[b] = load_octet(lst, o0 + kOctetBatch + (uint32_t)b, no);
 #pragma unroll
     for (int b = 0; b < kOctetBatch; b++) {
-      const uint32_t w[4] = {cur[b].x, cur[b].y, cur[b].z, cur[b].w};
+      if (o0 + (uint32_t)b < no) {
+        const uint32_t w[4] = {cur[b].x, cur[b].y, cur[b].z, cur[b].w};
+        double dx[8], dy[8], dz[8];
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
-        if ((o0 + (uint32_t)b) * 8u + (uint32_t)c < cnt) {
+        for (int c = 0; c < 8; c++) {
           const uint32_t e = (w[c >> 1] >> ((c & 1) * 16)) & 0xffffu;
-          double xj, yj, zj;
-          staged_pos(R, S, staged, e, B.x, B.y, B.z, xj, yj, zj);
-          const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
-          const double r2 = dist2(dx, dy, dz);
-          if (r2 < R.rc2) pair_accumulate<PREC, SF, ENERGIES>(R, dx, dy, dz, r2, a);
+          dx[c] = S.x[e] - xi;
+          dy[c] = S.y[e] - yi;
+          dz[c] = S.z[e] - zi;
         }
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+          pair_accumulate<PREC, SF, ENERGIES>(R, dx[c], dy[c], dz[c], dist2(dx[c], dy[c], dz[c]), a);
       }
     }
 #pragma unroll
     for (int b = 0; b < kOctetBatch; b++) cur[b] = nxt[b];
   }
   return a;
+}
+
+// The same sum for a brick whose neighbourhood did not fit in LDS: entry by
+// entry up to the count, positions through the cell tables.
+template <int PREC, bool SF, bool ENERGIES>
+__device__ __forceinline__ PairAcc interactions_global(const Buffers& B, const Replica& R, const Stage& S,
+                                                    uint32_t i, double xi, double yi, double zi) {
+  const uint32_t cnt = B.nbr_count[i];
+  const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
+  PairAcc a = {0.0, 0.0, 0.0, 0.0, 0.0, 0u, 0u};
+  for (uint32_t k = 0; k < cnt; k++) {
+    double xj, yj, zj;
+    staged_pos(R, S, false, lst[list_entry(k)], B.x, B.y, B.z, xj, yj, zj);
+    const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+    pair_accumulate<PREC, SF, ENERGIES>(R, dx, dy, dz, dist2(dx, dy, dz), a);
+  }
+  return a;
+}
+
+template <int PREC, bool SF, bool ENERGIES>
+__device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica& R, const Stage& S, bool staged,
+                                                uint32_t i, double xi, double yi, double zi) {
+  if (staged) return interactions_staged<PREC, SF, ENERGIES>(B, R, S, i, xi, yi, zi);
+  return interactions_global<PREC, SF, ENERGIES>(B, R, S, i, xi, yi, zi);
 }
 
 // New accelerations and (kick) the second half kick v += a dt/2, one
@@ -769,6 +814,133 @@ __global__ void __launch_bounds__(kBlock) k_measure(Buffers B) {
   block_add_slots(v, B.slots + (size_t)g.rep * kSlots);
 }
 
+// ---------------------------------------------------------------------------
+// Profiles along z (lennard_jones.wgsl:628-730). Slab b holds kProfileSums
+// sums from b * kProfileSums: particle count, v_a^2 (a = x, y, z), the
+// Irving-Kirkwood W_aa, potential energy; after the slabs, the whole-box
+// W_aa. Counts, kinetic terms and potential go to the slab of the atom's
+// wrapped z. A pair's half of the virial, w_a = (0.5 d_a) d_a (rf / r2), is
+// spread over the slabs that the segment from z_i to z_i + d_z crosses, in
+// proportion to the length inside each, the last slab taking what is left.
+// Every (pair, slab, axis) share and every per-atom value goes to fixed point
+// on its own as rint(x 2^32) and the integers are added, so the sums do not
+// depend on the order of the additions. Pair terms are FP64 in both precision
+// forms. Up to kProfileLdsBins slabs a workgroup adds into LDS and then once
+// per touched sum into the replica's sums; with more slabs it adds straight
+// into them.
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kProfileLdsBins = 128;
+constexpr double kProfileOne = 4294967296.0;  // fixed point: 2^32 per unit
+
+struct ProfileSink {
+  long long* lds;
+  long long* global;
+  bool use_lds;
+  __device__ __forceinline__ void add(uint32_t k, long long v) const {
+    if (v == 0) return;
+    if (use_lds)
+      atomicAdd((unsigned long long*)(lds + k), (unsigned long long)v);
+    else
+      atomicAdd((unsigned long long*)(global + k), (unsigned long long)v);
+  }
+  __device__ __forceinline__ void add(uint32_t k, double x) const { add(k, to_fixed(x, kProfileOne)); }
+};
+
+template <bool SF>
+__global__ void __launch_bounds__(kBlock) k_profile(Buffers B) {
+  const Brick g = B.bricks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const Replica R = B.rep[g.rep];
+  const uint32_t bins = R.prof_bins;
+  if (bins == 0) return;  // the whole workgroup leaves together
+  __shared__ Stage S;
+  __shared__ long long acc[kProfileLdsBins * kProfileSums + 3];
+  const uint32_t nsums = bins * kProfileSums + 3u;
+  const ProfileSink sink{acc, B.profile + R.prof_off, bins <= kProfileLdsBins};
+  if (sink.use_lds)
+    for (uint32_t k = threadIdx.x; k < nsums; k += kBlock) acc[k] = 0;
+  const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
+  const double width = R.bin_width;
+  const int ibins = (int)bins;
+  long long whole[3] = {0, 0, 0};
+  const uint32_t H = S.hoff[g.gx * g.gy];
+  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+    const Home at = home_atom(g, S, h);
+    const uint32_t i = at.slot;
+    double xi, yi, zi;
+    staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
+    const double zw = wrap_one(zi, R.box[2]);
+    const int hb = (int)floor(zw / width);
+    const uint32_t home = (uint32_t)(hb < 0 ? 0 : (hb > ibins - 1 ? ibins - 1 : hb));
+    sink.add(home * kProfileSums, 1.0);
+    const double v[3] = {B.vx[i], B.vy[i], B.vz[i]};
+#pragma unroll
+    for (int a = 0; a < 3; a++) sink.add(home * kProfileSums + 1u + (uint32_t)a, v[a] * v[a]);
+    double pe = 0.0, total[3] = {0.0, 0.0, 0.0};
+    const uint32_t cnt = B.nbr_count[i];
+    const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
+    for (uint32_t k = 0; k < cnt; k++) {
+      double xj, yj, zj;
+      staged_pos(R, S, staged, lst[list_entry(k)], B.x, B.y, B.z, xj, yj, zj);
+      const double d[3] = {xj - xi, yj - yi, zj - zi};
+      const double r2 = dist2(d[0], d[1], d[2]);
+      if (!(r2 < R.rc2)) continue;
+      const double inv2 = 1.0 / r2;
+      const double s6 = inv2 * inv2 * inv2;
+      double rf = s6 * __builtin_fma(48.0, s6, -24.0);
+      const double t = 4.0 * s6;
+      double u = __builtin_fma(t, s6, -t) - R.shift;
+      if (SF) {
+        const double r = __builtin_sqrt(r2);
+        rf = __builtin_fma(-r, R.force_shift, rf);
+        u = __builtin_fma(r - R.rc, R.force_shift, u);
+      }
+      pe += 0.5 * u;
+      const double q = rf / r2;
+      double w[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        w[a] = (0.5 * d[a]) * d[a] * q;
+        total[a] += w[a];
+      }
+      const double lo = zw + (d[2] < 0.0 ? d[2] : 0.0);
+      const double hi = zw + (d[2] > 0.0 ? d[2] : 0.0);
+      const double length = hi - lo;
+      if (length < 1e-6 * width) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) sink.add(home * kProfileSums + 4u + (uint32_t)a, w[a]);
+        continue;
+      }
+      int b = (int)floor(lo / width);
+      double left = 1.0;
+      // A segment shorter than the box crosses at most bins + 1 slabs.
+      for (uint32_t walk = 0; walk < bins + 2u; walk++) {
+        const bool last = (double)(b + 1) * width >= hi;
+        const double start = lo > (double)b * width ? lo : (double)b * width;
+        const double end = hi < (double)(b + 1) * width ? hi : (double)(b + 1) * width;
+        double share = last ? left : (end - start) / length;
+        share = share > 0.0 ? share : 0.0;
+        left -= share;
+        const uint32_t slab = (uint32_t)(((b % ibins) + ibins) % ibins);
+#pragma unroll
+        for (int a = 0; a < 3; a++) sink.add(slab * kProfileSums + 4u + (uint32_t)a, w[a] * share);
+        if (last) break;
+        b++;
+      }
+    }
+    sink.add(home * kProfileSums + 7u, pe);
+#pragma unroll
+    for (int a = 0; a < 3; a++) whole[a] += to_fixed(total[a], kProfileOne);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) sink.add(bins * kProfileSums + (uint32_t)a, whole[a]);
+  if (sink.use_lds) {
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nsums; k += kBlock)
+      if (acc[k] != 0) atomicAdd((unsigned long long*)(sink.global + k), (unsigned long long)acc[k]);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_zero_slots(Buffers B) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < B.n_replicas * kSlots) B.slots[t] = 0;
@@ -786,6 +958,15 @@ __global__ void __launch_bounds__(kBlock) k_fixed_sum(const double* values, size
 __global__ void k_zero_i64(long long* p, uint32_t n) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0;
+}
+
+// Plain streaming copy of n 16-byte words: reads and writes exactly 16 n
+// bytes. It calibrates the HBM byte counters and measures the bandwidth a
+// streaming kernel reaches on the device.
+__global__ void __launch_bounds__(kBlock) k_copy(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                                   size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + t), dst + t);
 }
 
 // ---------------------------------------------------------------------------
@@ -858,6 +1039,20 @@ hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStre
   return hipGetLastError();
 }
 
+hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s) {
+  if (!B.n_bricks) return hipSuccess;
+  if (shifted)
+    hipLaunchKernelGGL((k_profile<true>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+  else
+    hipLaunchKernelGGL((k_profile<false>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(long long* p, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_zero_i64, grid1((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fixed_sum(const double* values, size_t n, double scale, long long* out,
                             hipStream_t s) {
   hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(kBlock), 0, s, out, (uint32_t)kSlots);
@@ -865,6 +1060,12 @@ hipError_t launch_fixed_sum(const double* values, size_t n, double scale, long l
     const uint32_t blocks = (uint32_t)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_fixed_sum, grid1(blocks), dim3(kBlock), 0, s, values, n, scale, out);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t words, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)std::min<size_t>((words + kBlock - 1) / kBlock, 256u * 64u);
+  hipLaunchKernelGGL(k_copy, grid1(blocks), dim3(kBlock), 0, s, (const u32x4*)src, (u32x4*)dst, words);
   return hipGetLastError();
 }
 

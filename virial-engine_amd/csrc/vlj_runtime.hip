@@ -39,6 +39,7 @@ struct HostReplica {
   vlj_settings settings;
   double energy_scale, momentum_scale;
   uint32_t brick_x, brick_y;  // brick shape in cells
+  uint64_t prof_samples;      // profile samples since the last clear
 };
 
 }  // namespace
@@ -53,6 +54,7 @@ struct vlj_sim {
   std::vector<Brick> bricks;
   uint32_t n_total = 0, cells_total = 0;
   uint64_t list_total = 0;
+  uint64_t prof_total = 0;  // profile sums over all replicas
   int precision = 0;
   bool shifted = false;
   bool any_langevin = false;
@@ -215,6 +217,13 @@ int derive(const vlj_particle* p, size_t n, const vlj_settings& st, Replica& R, 
   }
   R.seed = st.thermostat.seed;
   R.near_count = 4;
+  if (st.profile_bins > kMaxProfileBins) {
+    err = "profile_bins must be at most " + std::to_string(kMaxProfileBins);
+    return VLJ_E_INVALID;
+  }
+  R.prof_bins = st.profile_bins;
+  R.bin_width = R.box[2] / (double)(st.profile_bins ? st.profile_bins : 1u);
+  H.prof_samples = 0;
   return VLJ_OK;
 }
 
@@ -358,7 +367,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   s->precision = (int)settings[0].precision;
   s->shifted = settings[0].shifted_force != 0;
   uint32_t atom_off = 0, cell_off = 0, scan_off = 0;
-  uint64_t list_off = 0;
+  uint64_t list_off = 0, prof_off = 0;
   for (size_t r = 0; r < replicas; r++) {
     Replica& R = s->rep[r];
     std::string err;
@@ -375,6 +384,12 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     R.scan_block_off = scan_off;
     R.scan_blocks = (R.total_cells + kScanCells - 1) / kScanCells;
     R.list_off = list_off;
+    R.prof_off = (uint32_t)prof_off;
+    if (R.prof_bins) prof_off += (uint64_t)R.prof_bins * kProfileSums + 3u;
+    if (prof_off >= (1ull << 31)) {
+      delete s;
+      return fail(nullptr, VLJ_E_INVALID, "the profiles of the batch need too many sums");
+    }
     R.atom_block_off = (uint32_t)s->atom_blocks.size();
     R.atom_blocks = (R.count + kBlock - 1) / kBlock;
     choose_brick(R, occ, s->host[r].brick_x, s->host[r].brick_y);
@@ -403,6 +418,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   s->n_total = atom_off;
   s->cells_total = cell_off;
   s->list_total = list_off;
+  s->prof_total = prof_off;
   if (list_off >= (1ull << 40)) {
     delete s;
     return fail(nullptr, VLJ_E_INVALID, "neighbour lists would not fit in device memory");
@@ -447,6 +463,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   A(&B.bprefix, s->scan_blocks.size());
   A(&B.nbr, s->list_total);
   A(&B.slots, replicas * kSlots);
+  A(&B.profile, std::max<uint64_t>(s->prof_total, 1));
   A(&B.tickets, 4);
   A(&s->fixed_scratch, kSlots);
   if (rc != VLJ_OK) {
@@ -501,6 +518,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   if (e == hipSuccess) e = hipMemset(B.ay, 0, std::max<size_t>(N, 1) * 8);
   if (e == hipSuccess) e = hipMemset(B.az, 0, std::max<size_t>(N, 1) * 8);
   if (e == hipSuccess) e = hipMemset(B.nbr_count, 0, std::max<size_t>(N, 1) * 4);
+  if (e == hipSuccess) e = hipMemset(B.profile, 0, std::max<uint64_t>(s->prof_total, 1) * 8);
   for (auto& r : s->ring) {
     if (e == hipSuccess) e = hipHostMalloc((void**)&r.slots, replicas * kSlots * sizeof(long long), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&r.ctl, replicas * sizeof(Control), hipHostMallocDefault);
@@ -768,26 +786,74 @@ int vlj_set_condensed_neighbours(vlj_sim* sim, uint32_t neighbours) {
   return VLJ_OK;
 }
 
+static bool any_profiles(const vlj_sim* s) { return s->prof_total != 0; }
+
+// lennard_jones.rs:1167-1183: prime if needed, then one sample added to the
+// running sums of every replica with profile slabs. Asynchronous.
 int vlj_sample_profile(vlj_sim* sim) {
   if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
-  if (sim->host[0].settings.profile_bins == 0)
-    return fail(sim, VLJ_E_STATE, "profiles need Settings::profile_bins above zero");
-  return fail(sim, VLJ_E_UNSUPPORTED, "profiles are not built yet");
-}
-
-int vlj_clear_profile(vlj_sim* sim) {
-  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (!any_profiles(sim)) return fail(sim, VLJ_E_STATE, "profiles need Settings::profile_bins above zero");
+  int rc = prime_if_needed(sim);
+  if (rc != VLJ_OK) return rc;
+  HIP_TRY(sim, launch_profile(sim->B, sim->shifted, sim->stream));
+  for (auto& h : sim->host)
+    if (h.settings.profile_bins) h.prof_samples++;
   return VLJ_OK;
 }
 
-int vlj_read_profile(vlj_sim* sim, double* sums, size_t capacity, uint64_t* samples) {
-  (void)sums;
-  (void)capacity;
-  (void)samples;
+// lennard_jones.rs:1186-1191. Without profile slabs there is nothing to clear.
+int vlj_clear_profile(vlj_sim* sim) {
   if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
-  if (sim->host[0].settings.profile_bins == 0)
-    return fail(sim, VLJ_E_STATE, "profiles need Settings::profile_bins above zero");
-  return fail(sim, VLJ_E_UNSUPPORTED, "profiles are not built yet");
+  if (!any_profiles(sim)) return VLJ_OK;
+  HIP_TRY(sim, launch_zero(sim->B.profile, (uint32_t)sim->prof_total, sim->stream));
+  for (auto& h : sim->host) h.prof_samples = 0;
+  return VLJ_OK;
+}
+
+static int read_profile_sums(vlj_sim* sim, size_t replica, std::vector<long long>& raw, uint64_t* samples) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (replica >= sim->rep.size()) return fail(sim, VLJ_E_INVALID, "no such replica");
+  const Replica& R = sim->rep[replica];
+  if (R.prof_bins == 0) return fail(sim, VLJ_E_STATE, "profiles need Settings::profile_bins above zero");
+  raw.assign((size_t)R.prof_bins * kProfileSums + 3u, 0);
+  int rc = read_dev(sim, raw.data(), sim->B.profile + R.prof_off, raw.size() * sizeof(long long));
+  if (rc != VLJ_OK) return rc;
+  if (samples) *samples = sim->host[replica].prof_samples;
+  return VLJ_OK;
+}
+
+static int copy_raw(vlj_sim* sim, const std::vector<long long>& raw, int64_t* out, size_t capacity) {
+  if (!out || capacity < raw.size())
+    return fail(sim, VLJ_E_INVALID, "the profile needs profile_bins * 8 + 3 values");
+  std::memcpy(out, raw.data(), raw.size() * sizeof(long long));
+  return VLJ_OK;
+}
+
+// lennard_jones.rs:1297-1325: the sums decoded from fixed point and divided
+// by the sample count (NaN with no samples, as 0 / 0 there).
+int vlj_read_profile(vlj_sim* sim, double* out, size_t capacity, uint64_t* samples) {
+  std::vector<long long> raw;
+  uint64_t n = 0;
+  int rc = read_profile_sums(sim, 0, raw, &n);
+  if (rc != VLJ_OK) return rc;
+  if (!out || capacity < raw.size())
+    return fail(sim, VLJ_E_INVALID, "the profile needs profile_bins * 8 + 3 values");
+  for (size_t k = 0; k < raw.size(); k++) out[k] = ((double)raw[k] / kTwo32) / (double)n;
+  if (samples) *samples = n;
+  return VLJ_OK;
+}
+
+int vlj_read_profile_raw(vlj_sim* sim, int64_t* raw_out, size_t capacity, uint64_t* samples) {
+  std::vector<long long> raw;
+  int rc = read_profile_sums(sim, 0, raw, samples);
+  return rc != VLJ_OK ? rc : copy_raw(sim, raw, raw_out, capacity);
+}
+
+int vlj_batch_read_profile_raw(vlj_batch* sim, size_t replica, int64_t* raw_out, size_t capacity,
+                               uint64_t* samples) {
+  std::vector<long long> raw;
+  int rc = read_profile_sums(sim, replica, raw, samples);
+  return rc != VLJ_OK ? rc : copy_raw(sim, raw, raw_out, capacity);
 }
 
 const char* vlj_last_error(const vlj_sim* sim) { return sim ? sim->error.c_str() : g_error.c_str(); }
@@ -955,6 +1021,38 @@ int vlj_debug_params(const vlj_sim* sim, double out[6]) {
 }
 
 uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks) { return xcd_swizzle(block, blocks); }
+
+int vlj_copy_bandwidth(size_t bytes, uint32_t repeats, double* ms) {
+  if (!ms || bytes < 16 || repeats == 0) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_copy_bandwidth");
+  vlj_sim tmp;
+  int rc = check_device(&tmp);
+  if (rc != VLJ_OK) return rc;
+  const size_t words = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&a, words * 16);
+  if (e == hipSuccess) e = hipMalloc(&b, words * 16);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, words * 16, st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_copy(a, b, words, st);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (uint32_t k = 0; k < repeats && e == hipSuccess; k++) e = launch_copy(a, b, words, st);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (e != hipSuccess) return fail(nullptr, VLJ_E_HIP, std::string("vlj_copy_bandwidth: ") + hipGetErrorString(e));
+  *ms = (double)t / repeats;
+  return VLJ_OK;
+}
 
 int vlj_timed_run(vlj_sim* sim, uint64_t steps, double* force_ms, double* total_ms) {
   if (!sim || !force_ms || !total_ms) return fail(sim, VLJ_E_INVALID, "bad arguments to vlj_timed_run");

@@ -99,6 +99,8 @@ SIGNATURES = {
     "vlj_sample_profile": (C.c_int, [_P]),
     "vlj_clear_profile": (C.c_int, [_P]),
     "vlj_read_profile": (C.c_int, [_P, _dp, _sz, _u64p]),
+    "vlj_read_profile_raw": (C.c_int, [_P, _i64p, _sz, _u64p]),
+    "vlj_batch_read_profile_raw": (C.c_int, [_P, _sz, _i64p, _sz, _u64p]),
     "vlj_last_error": (C.c_char_p, [_P]),
     "vlj_batch_create": (C.c_int, [_pp, _u64p, C.POINTER(Settings), _sz, C.POINTER(_P)]),
     "vlj_batch_run": (C.c_int, [_P, _u64]),
@@ -115,6 +117,7 @@ SIGNATURES = {
     "vlj_debug_grid": (C.c_int, [_P, _up]),
     "vlj_debug_params": (C.c_int, [_P, _dp]),
     "vlj_debug_swizzle": (_u32, [_u32, _u32]),
+    "vlj_copy_bandwidth": (C.c_int, [_sz, _u32, _dp]),
     "vlj_timed_run": (C.c_int, [_P, _u64, _dp, _dp]),
 }
 

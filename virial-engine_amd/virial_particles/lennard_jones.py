@@ -12,6 +12,8 @@ theorem, P = (2 KE + W) / (3 V) (lennard_jones.rs:110-131).
 from __future__ import annotations
 
 import ctypes as C
+import enum
+import math
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Union
 
@@ -142,6 +144,129 @@ class Measurement:
 
     def extended_energy(self) -> float:
         return self.energy() + self.thermostat_energy
+
+
+PROFILE_SUMS = 8  # sums per slab in the profile buffer (lennard_jones.rs:321)
+
+
+@dataclass
+class Slab:
+    """One slab of a profile, averaged over samples (lennard_jones.rs:133-147)."""
+
+    z: float
+    particles: float
+    kinetic: List[float]   # sum of m v_a^2 over the slab's particles
+    virial: List[float]    # diagonal of the virial tensor, spread along each pair's line (Irving-Kirkwood)
+    potential: float       # half of each pair's energy to each particle's slab
+
+
+class Phase(enum.Enum):
+    """Where a slab sits in a liquid-vapour profile (lennard_jones.rs:149-156)."""
+
+    LIQUID = "liquid"
+    VAPOUR = "vapour"
+    INTERFACE = "interface"
+    BETWEEN = "between"
+
+
+def _round_half_away(x: float) -> float:
+    # f64::round: halves away from zero
+    return math.copysign(math.floor(abs(x) + 0.5), x)
+
+
+@dataclass
+class Profile:
+    """Profiles along z, averaged over samples (lennard_jones.rs:158-271);
+    from LennardJones.read_profile."""
+
+    samples: int
+    box_size: List[float]
+    slabs: List[Slab]
+    virial: List[float]    # whole-box virial diagonal, summed pair by pair
+
+    @classmethod
+    def from_sums(cls, sums: Sequence[float], samples: int, box_size: Sequence[float]) -> "Profile":
+        """From the profile_bins * 8 + 3 averaged sums of vlj_read_profile
+        (lennard_jones.rs:1297-1325)."""
+        bins = (len(sums) - 3) // PROFILE_SUMS
+        width = box_size[2] / bins
+        slabs = []
+        for b in range(bins):
+            q = sums[b * PROFILE_SUMS:(b + 1) * PROFILE_SUMS]
+            slabs.append(Slab((b + 0.5) * width, float(q[0]), [float(q[1]), float(q[2]), float(q[3])],
+                              [float(q[4]), float(q[5]), float(q[6])], float(q[7])))
+        t = sums[bins * PROFILE_SUMS:]
+        return cls(int(samples), [float(x) for x in box_size], slabs, [float(t[0]), float(t[1]), float(t[2])])
+
+    def slab_volume(self) -> float:
+        lx, ly, lz = self.box_size
+        return lx * ly * lz / len(self.slabs)
+
+    def density(self) -> List[float]:
+        """Number density in each slab."""
+        v = self.slab_volume()
+        return [s.particles / v for s in self.slabs]
+
+    def temperature(self) -> List[float]:
+        """Kinetic temperature in each slab (3 degrees of freedom per particle)."""
+        return [sum(s.kinetic) / (3.0 * s.particles) if s.particles else math.nan for s in self.slabs]
+
+    def pressure_tensor(self) -> List[List[float]]:
+        """Diagonal (P_xx, P_yy, P_zz) of the pressure tensor in each slab."""
+        v = self.slab_volume()
+        return [[(s.kinetic[a] + s.virial[a]) / v for a in range(3)] for s in self.slabs]
+
+    def total_pressure_tensor(self) -> List[float]:
+        """Diagonal of the whole box's pressure tensor."""
+        volume = self.box_size[0] * self.box_size[1] * self.box_size[2]
+        return [(sum(s.kinetic[a] for s in self.slabs) + self.virial[a]) / volume for a in range(3)]
+
+    def interfaces(self, midpoint: float) -> List[float]:
+        """Heights where the density crosses `midpoint`, interpolated between
+        slab centres around the periodic z axis."""
+        density = self.density()
+        n = len(density)
+        width = self.box_size[2] / n
+        out = []
+        for b in range(n):
+            a, c = density[b] - midpoint, density[(b + 1) % n] - midpoint
+            if (a < 0.0) != (c < 0.0):
+                z = self.slabs[b].z + width * a / (a - c)
+                out.append(z % self.box_size[2])
+        return out
+
+    def phases(self, midpoint: float, margin: float) -> List[Phase]:
+        """Liquid or vapour at least `margin` from every interface, interface
+        within margin / 2 of one, between otherwise; all between without an
+        interface."""
+        lz = self.box_size[2]
+        interfaces = self.interfaces(midpoint)
+        out = []
+        for s, rho in zip(self.slabs, self.density()):
+            distance = math.inf
+            for z in interfaces:
+                d = s.z - z
+                distance = min(distance, abs(d - lz * _round_half_away(d / lz)))
+            if math.isinf(distance):
+                out.append(Phase.BETWEEN)
+            elif distance >= margin:
+                out.append(Phase.LIQUID if rho > midpoint else Phase.VAPOUR)
+            elif distance <= margin / 2.0:
+                out.append(Phase.INTERFACE)
+            else:
+                out.append(Phase.BETWEEN)
+        return out
+
+    def mean_in(self, values: Sequence[float], phases: Sequence[Phase], phase: Phase) -> Optional[float]:
+        """Mean of a per-slab quantity over the slabs in `phase`; None if none."""
+        picked = [v for v, p in zip(values, phases) if p == phase]
+        return sum(picked) / len(picked) if picked else None
+
+    def surface_tension(self) -> float:
+        """gamma = L_z / 2 (P_zz - (P_xx + P_yy) / 2) for a slab spanning x
+        and y, the 2 for its two interfaces."""
+        pxx, pyy, pzz = self.total_pressure_tensor()
+        return 0.5 * self.box_size[2] * (pzz - 0.5 * (pxx + pyy))
 
 
 def _as_particle_array(particles) -> np.ndarray:
@@ -293,6 +418,24 @@ class LennardJones:
     def clear_profile(self) -> None:
         self._c(L.lib().vlj_clear_profile(self._h))
 
+    def _profile_size(self) -> int:
+        return int(self._settings.profile_bins) * PROFILE_SUMS + 3
+
+    def read_profile(self) -> Profile:
+        """The profiles averaged over every sample since the last clear."""
+        out = np.zeros(self._profile_size())
+        samples = C.c_uint64(0)
+        self._c(L.lib().vlj_read_profile(self._h, _dptr(out), out.shape[0], C.byref(samples)))
+        return Profile.from_sums(out, samples.value, self.box_size())
+
+    def read_profile_raw(self):
+        """The 64-bit fixed-point sums (value * 2^32) and the sample count."""
+        out = np.zeros(self._profile_size(), dtype=np.int64)
+        samples = C.c_uint64(0)
+        self._c(L.lib().vlj_read_profile_raw(self._h, out.ctypes.data_as(C.POINTER(C.c_int64)), out.shape[0],
+                                             C.byref(samples)))
+        return out, samples.value
+
     # ---- inspection (parity tests; no counterpart in the reference)
     def debug_state(self):
         n = self.count()
@@ -353,6 +496,7 @@ class Batch:
                                         len(replicas), C.byref(h)))
         self._h = h
         self.replicas = len(replicas)
+        self.profile_bins = [int(st.profile_bins) for _, st in replicas]
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -377,6 +521,16 @@ class Batch:
         out = np.zeros(sum(self.sizes), dtype=PARTICLE_DTYPE)
         _check(L.lib().vlj_batch_read_particles(self._h, _pptr(out), out.shape[0]), self._h)
         return out
+
+    def sample_profile(self) -> None:
+        _check(L.lib().vlj_sample_profile(self._h), self._h)
+
+    def read_profile_raw(self, replica: int):
+        out = np.zeros(self.profile_bins[replica] * PROFILE_SUMS + 3, dtype=np.int64)
+        samples = C.c_uint64(0)
+        _check(L.lib().vlj_batch_read_profile_raw(self._h, replica, out.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                  out.shape[0], C.byref(samples)), self._h)
+        return out, samples.value
 
 
 def lattice(count: int, box_size: float, temperature: float, seed: int) -> List[Particle]:
@@ -408,6 +562,13 @@ def fcc_array(cells_per_side: int, density: float, temperature: float, seed: int
     side = C.c_double()
     _check(L.lib().vlj_fcc(cells_per_side, density, temperature, seed, _pptr(out), C.byref(side)))
     return out, side.value
+
+
+def copy_bandwidth(nbytes: int, repeats: int = 10) -> float:
+    """Average ms of a plain device copy of nbytes (read once, written once)."""
+    ms = C.c_double()
+    _check(L.lib().vlj_copy_bandwidth(nbytes, repeats, C.byref(ms)))
+    return ms.value
 
 
 def fixed_sum(values, fixed_scale: float) -> int:
