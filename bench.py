@@ -5,7 +5,10 @@ momentum), rc 2.5 truncated and shifted, Verlet lists with skin 0.3 rebuilt
 at skin/2, velocity Verlet dt 0.005, NVE; int64 fixed-point energy and virial.
 Timed: K steps of the plain FP64 form (the headline `value`), then K steps of
 the FP32-pair-force form from the same initial state. A step includes the
-rebuild decision and every list rebuild that falls in the window.
+rebuild decision and every list rebuild that falls in the window. Then the
+batched form: 512 Langevin replicas of mixed size, density and temperature
+in the same launches, 5000 steps, with each replica's PV/NkT and its
+standard error from block means.
 
 Also reported:
   - the force kernel's roofline: algorithmic bytes per launch over its
@@ -16,6 +19,8 @@ Also reported:
 
 Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W]
+  python bench.py --precision-study --cells 128 --steps 10000  (FP32 vs FP64 forms, 8M atoms)
+  python bench.py --slab --steps 2000                          (8 seeds of the 2M-atom slab)
   python bench.py --summarize ROCPROF_DIR --out FILE     (text summary of a rocprofv3 run)
   python bench.py --traffic FETCH_DIR WRITE_DIR --out FILE (calibrated bytes per force launch)
 """
@@ -70,6 +75,199 @@ def run_form(lj, particles, side, precision, steps, warmup, timed_steps):
         "energy_end": end.energy(),
         "list_overflow": end.list_overflow,
     }
+
+
+def batch_workload(lj, replicas, seed):
+    """The batched-replica workload: N log-uniform in [500, 50000], rho
+    uniform in [0.01, 0.85], T uniform in [0.7, 2.0]; FCC sites for rho >= 0.3,
+    random insertion 0.9 apart below; Langevin gamma 1 with a seed per
+    replica; rc 2.5, skin 0.3, dt 0.005, FP64."""
+    import math
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    specs, meta = [], []
+    for _ in range(replicas):
+        n = int(round(math.exp(rng.uniform(math.log(500.0), math.log(50000.0)))))
+        rho = float(rng.uniform(0.01, 0.85))
+        temp = float(rng.uniform(0.7, 2.0))
+        rseed = int(rng.integers(1, 1 << 62))
+        if rho >= 0.3:
+            parts, side = lj.fcc_count_array(n, rho, temp, rseed)
+        else:
+            side = float(np.float32((n / rho) ** (1.0 / 3.0)))
+            parts = lj.random_insertion_array(n, [side] * 3, 0.9, temp, rseed)
+        st = lj.Settings([side] * 3, 2.5, 0.005, skin=0.3, thermostat=lj.Langevin(temp, 1.0, rseed))
+        specs.append((parts, st))
+        meta.append({"n": n, "rho": n / side ** 3, "T": temp})
+    return specs, meta
+
+
+def run_batch(lj, specs, meta, steps, warmup, block_steps, blocks):
+    """Runs the batch; times the stepping only. Pressure is sampled every
+    block_steps steps and PV/NkT gets a standard error from `blocks` block
+    means (tests/thermodynamics.rs:259-277)."""
+    import numpy as np
+
+    b = lj.Batch(specs)
+    b.run(warmup)
+    samples = []
+    stepping = 0.0
+    for _ in range(max(steps // block_steps, 1)):
+        t0 = time.perf_counter()
+        b.run(block_steps)
+        stepping += time.perf_counter() - t0
+        ms = b.measure_now()
+        samples.append([m.pressure() * m.volume / (m.count * meta[r]["T"]) for r, m in enumerate(ms)])
+    last = b.measure_now()
+    b.close()
+    z = np.array(samples)  # samples x replicas
+    usable = (z.shape[0] // blocks) * blocks
+    means = z[:usable].reshape(blocks, -1, z.shape[1]).mean(axis=1)
+    zmean = means.mean(axis=0)
+    zse = means.std(axis=0, ddof=1) / np.sqrt(blocks)
+    atoms = sum(m["n"] for m in meta)
+    done = max(steps // block_steps, 1) * block_steps
+    return {
+        "replicas": len(specs),
+        "atoms": atoms,
+        "steps": done,
+        "seconds_stepping": stepping,
+        "atom_steps_per_s": atoms * done / stepping,
+        "pairs_per_s": sum(m.pairs_within_cutoff for m in last) * done / stepping,
+        "pv_nkt": [{"n": m["n"], "rho": m["rho"], "T": m["T"], "z": float(zmean[r]), "se": float(zse[r])}
+                   for r, m in enumerate(meta)],
+        "list_overflow": int(max(m.list_overflow for m in last)),
+    }
+
+
+def precision_study(lj, cells, steps, every):
+    """The FP32-pair-force form against the FP64 form (bit-exact with the f64
+    oracle) on FCC 4 x cells^3 at rho 0.8442, T 0.722: per-atom force error
+    at the first step, PE and virial error, and NVE energy drift of both over
+    `steps` steps, sampled every `every` steps."""
+    import numpy as np
+
+    particles, side = lj.fcc_array(cells, 0.8442, 0.722, 2024)
+    n = particles.shape[0]
+    out = {"workload": f"FCC 4x{cells}^3 N={n} rho=0.8442 T=0.722 rc=2.5 skin=0.3 dt=0.005 NVE", "n": n,
+           "steps": steps}
+    forms = {}
+    for label, prec in (("fp64", 0), ("fp32", 1)):
+        sim = lj.LennardJones(particles, lj.Settings([side] * 3, 2.5, 0.005, skin=0.3, precision=prec))
+        m0 = sim.measure_now()
+        _, _, acc, orig = sim.debug_state()
+        a = np.zeros_like(acc)
+        a[orig] = acc
+        energies = [(0, m0.energy())]
+        t0 = time.perf_counter()
+        for k in range(steps // every):
+            sim.run(every)
+            energies.append(((k + 1) * every, sim.measure_now().energy()))
+        seconds = time.perf_counter() - t0
+        sim.close()
+        e = np.array([x[1] for x in energies])
+        forms[label] = {"m0": m0, "acc": a, "energies": energies, "seconds": seconds,
+                        "drift": float((e[-1] - e[0]) / abs(e[0])),
+                        "max_abs_deviation": float(np.abs(e - e[0]).max() / abs(e[0]))}
+        log(f"{label}: drift {forms[label]['drift']:.3e}, max |dE|/|E0| {forms[label]['max_abs_deviation']:.3e}, "
+            f"{seconds:.1f} s")
+    a64, a32 = forms["fp64"]["acc"], forms["fp32"]["acc"]
+    err = np.linalg.norm(a32 - a64, axis=1)
+    mag = np.linalg.norm(a64, axis=1)
+    m64, m32 = forms["fp64"]["m0"], forms["fp32"]["m0"]
+    out.update({
+        "force_rms_relative": float(np.sqrt(np.mean(err ** 2) / np.mean(mag ** 2))),
+        "force_max_relative_to_rms": float(err.max() / np.sqrt(np.mean(mag ** 2))),
+        "force_median_per_atom_relative": float(np.median(err / np.maximum(mag, 1e-300))),
+        "pe_relative": float((m32.potential - m64.potential) / abs(m64.potential)),
+        "virial_relative": float((m32.virial - m64.virial) / abs(m64.virial)),
+        "drift_fp64": forms["fp64"]["drift"],
+        "drift_fp32": forms["fp32"]["drift"],
+        "max_deviation_fp64": forms["fp64"]["max_abs_deviation"],
+        "max_deviation_fp32": forms["fp32"]["max_abs_deviation"],
+        "energy_fp64": forms["fp64"]["energies"][:: max(1, len(forms["fp64"]["energies"]) // 10)],
+        "energy_fp32": forms["fp32"]["energies"][:: max(1, len(forms["fp32"]["energies"]) // 10)],
+        "atom_steps_per_s_fp64": n * steps / forms["fp64"]["seconds"],
+        "atom_steps_per_s_fp32": n * steps / forms["fp32"]["seconds"],
+    })
+    return out
+
+
+def slab_replica(lj, cells, seed):
+    """The liquid-vapour slab: an FCC liquid block of cells^3 unit cells at
+    rho ~0.73 spanning the middle quarter of a box with Lz = 4 Lx, random
+    vapour at rho 0.02 (0.9 apart, 1 sigma clear of the liquid) elsewhere;
+    T 0.7 velocities."""
+    import numpy as np
+
+    liquid, lx = lj.fcc_array(cells, 0.73, 0.7, seed)
+    lz = 4.0 * lx
+    liquid["pos"][:, 2] += 1.5 * lx  # the block fills z in [1.5 Lx, 2.5 Lx)
+    gap = 1.0
+    height = 3.0 * lx - 2.0 * gap
+    n_vap = int(round(0.02 * lx * lx * height))
+    vapour = lj.random_insertion_array(n_vap, [lx, lx, height], 0.9, 0.7, seed + 1)
+    vapour["pos"][:, 2] = np.mod(vapour["pos"][:, 2] + 2.5 * lx + gap, lz)
+    parts = np.concatenate([liquid, vapour])
+    vel = parts["vel"] - parts["vel"].mean(axis=0)
+    parts["vel"] = vel
+    return parts, [lx, lx, lz]
+
+
+def run_slab(lj, cells, seeds, steps, warmup, sample_every, bins):
+    """8 seeds of the slab in one batch (force-shifted rc 2.5, Langevin gamma
+    0.5 at T 0.7, FP32 pair forces, `bins` z slabs); times the stepping and
+    samples the profiles every `sample_every` steps."""
+    import numpy as np
+
+    specs = []
+    for k in range(seeds):
+        parts, box = slab_replica(lj, cells, 100 + 2 * k)
+        st = lj.Settings(box, 2.5, 0.005, skin=0.3, shifted_force=True, profile_bins=bins, precision=1,
+                         thermostat=lj.Langevin(0.7, 0.5, 1000 + k))
+        specs.append((parts, st))
+    atoms = sum(p.shape[0] for p, _ in specs)
+    b = lj.Batch(specs)
+    b.run(warmup)
+    stepping = 0.0
+    done = 0
+    while done < steps:
+        k = min(sample_every, steps - done)
+        t0 = time.perf_counter()
+        b.run(k)
+        stepping += time.perf_counter() - t0
+        b.sample_profile()
+        done += k
+    last = b.measure_now()
+    out = {"replicas": seeds, "atoms": atoms, "steps": steps, "seconds_stepping": stepping,
+           "atom_steps_per_s": atoms * steps / stepping,
+           "pairs_per_s": sum(m.pairs_within_cutoff for m in last) * steps / stepping}
+    tensions, rho_l, rho_v = [], [], []
+    for r in range(seeds):
+        raw, samples = b.read_profile_raw(r)
+        sums = raw / 2.0 ** 32 / samples
+        prof = lj.Profile.from_sums(sums, samples, specs[r][1].box_size)
+        phases = prof.phases(0.36, 4.0)
+        dens = prof.density()
+        rho_l.append(prof.mean_in(dens, phases, lj.Phase.LIQUID))
+        rho_v.append(prof.mean_in(dens, phases, lj.Phase.VAPOUR))
+        tensions.append(prof.surface_tension())
+    b.close()
+
+    def mean_se(v):
+        v = np.array([x for x in v if x is not None], dtype=float)
+        return (float(v.mean()), float(v.std(ddof=1) / np.sqrt(len(v)))) if len(v) > 1 else (None, None)
+
+    out["surface_tension"] = mean_se(tensions)
+    out["liquid_density"] = mean_se(rho_l)
+    out["vapour_density"] = mean_se(rho_v)
+    out["profile_samples"] = int(steps // sample_every)
+    out["workload"] = (f"{seeds} seeds of a liquid-vapour slab, Lz = 4 Lx, FCC liquid (rho 0.725) in the middle "
+                       f"quarter, vapour rho 0.02; force-shifted rc 2.5, Langevin gamma 0.5 T 0.7, FP32 pair "
+                       f"forces, {bins} z slabs")
+    return out
 
 
 def cpu_baseline(n_side, density, temperature, seed, steps, single_steps):
@@ -193,6 +391,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-single-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="FP32 vs FP64 forms on FCC 4 x cells^3 (use --cells 128 --steps 10000)")
+    ap.add_argument("--measure-every", type=int, default=500)
+    ap.add_argument("--slab", action="store_true", help="8 seeds of the 2M-atom liquid-vapour slab")
+    ap.add_argument("--slab-cells", type=int, default=77, help="FCC cells per side of the slab's liquid block")
+    ap.add_argument("--batch-replicas", type=int, default=512, help="0 skips the batched form")
+    ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--batch-table", help="write each replica's PV/NkT and standard error here (JSON)")
     ap.add_argument("--summarize", metavar="ROCPROF_DIR")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--out")
@@ -210,6 +416,13 @@ def main():
         sys.exit(2)
 
     from virial_particles import lennard_jones as lj
+
+    if args.slab:
+        print(json.dumps(run_slab(lj, args.slab_cells, 8, args.steps, args.warmup, 100, 200)), flush=True)
+        return
+    if args.precision_study:
+        print(json.dumps(precision_study(lj, args.cells, args.steps, args.measure_every)), flush=True)
+        return
 
     copy_ms = lj.copy_bandwidth(COPY_BYTES, 10)
     copy_gbs = 2.0 * COPY_BYTES / (copy_ms * 1e-3) / 1e9
@@ -241,6 +454,21 @@ def main():
         "avg_launch_us": fp64["force_kernel_us"],
         "measured_copy_GBs": copy_gbs,
     }
+    batched = None
+    if args.batch_replicas > 0:
+        specs, meta = batch_workload(lj, args.batch_replicas, 7)
+        br = run_batch(lj, specs, meta, args.batch_steps, args.warmup, 25, 20)
+        zs = br.pop("pv_nkt")
+        if args.batch_table:
+            json.dump(zs, open(args.batch_table, "w"), indent=0)
+        se_rel = sorted(z["se"] / abs(z["z"]) for z in zs if z["z"] != 0)
+        batched = dict(br)
+        batched["ratio_vs_single_fp64"] = br["atom_steps_per_s"] / fp64["atom_steps_per_s"]
+        batched["pv_nkt_median_relative_se"] = se_rel[len(se_rel) // 2] if se_rel else None
+        batched["workload"] = (f"{args.batch_replicas} Langevin replicas (gamma 1, T 0.7-2.0), N 500-50000 "
+                               f"log-uniform, rho 0.01-0.85, FCC or random insertion, rc 2.5, dt 0.005, FP64; "
+                               f"pressure every 25 steps, 20 blocks")
+        log(f"batched: {json.dumps(batched)}")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cells, density, temperature, seed, args.cpu_steps, args.cpu_single_steps)
@@ -274,6 +502,7 @@ def main():
         "pairs_per_s": fp64["pairs_within_cutoff"] * args.steps / fp64["seconds"],
         "rebuilds_in_window": fp64["rebuilds"],
         "energy_drift_in_window": (fp64["energy_end"] - fp64["energy_start"]) / abs(fp64["energy_start"]),
+        "batched": batched,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -186,6 +186,18 @@ int vlj_lattice_box(size_t count, const double size[3], double temperature, uint
  * *side = the cube's side, already rounded to f32. */
 int vlj_fcc(uint32_t cells_per_side, double density, double temperature, uint64_t seed,
             vlj_particle *out, double *side);
+/* FCC sites for any count (no counterpart in the reference): the smallest
+ * 4 n^3 lattice holding `count` atoms, filled in (cz, cy, cx, basis) order,
+ * in a cube of side cbrt(count / density) rounded to f32 (written to *side). */
+int vlj_fcc_count(size_t count, double density, double temperature, uint64_t seed, vlj_particle *out,
+                  double *side);
+/* Random non-overlapping insertion (no counterpart in the reference):
+ * uniform trials from the xorshift stream, rejected within min_distance
+ * (nearest image) of an earlier particle, 1000 trials per particle at most
+ * (VLJ_E_INVALID when they do not suffice); Gaussian velocities from the
+ * same stream at `temperature`, mean removed. */
+int vlj_random_insertion(size_t count, const double box[3], double min_distance, double temperature,
+                         uint64_t seed, vlj_particle *out);
 
 /* ---- fixed point (crates/virial-precision/src/fixed.rs:9-57) ----
  * Device reduction used by every measurement: sum of round-ties-even(values[i]
@@ -215,6 +227,10 @@ int vlj_debug_params(const vlj_sim *sim, double out[6]);
  * contiguous share of the table. A bijection on [0, blocks). */
 uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks);
 
+/* The force loop's reciprocal 1 / x for x in [1e-4, 1e37] (n values), for
+ * the check that it rounds as IEEE division does. No counterpart in the
+ * reference. */
+int vlj_debug_rcp(const double *x, double *out, size_t n);
 /* Plain streaming copy of `bytes` (read once, written once) between two
  * device buffers, `repeats` times; *ms = average kernel time from HIP
  * events. Measures the bandwidth a streaming kernel reaches and calibrates

@@ -45,6 +45,9 @@ class Params(C.Structure):
         ("precision", C.c_int32),
         ("shifted_force", C.c_int32),
         ("near_count", C.c_uint32),
+        ("nh_gkt", C.c_double),
+        ("nh_q", C.c_double),
+        ("nh_period", C.c_double),
     ]
 
 
@@ -62,6 +65,7 @@ class Measurement(C.Structure):
         ("condensed", C.c_uint32),
         ("pairs_within_cutoff", C.c_uint64),
         ("raw", C.c_int64 * 8),
+        ("thermostat_energy", C.c_double),
     ]
 
 
@@ -98,9 +102,10 @@ def lib() -> C.CDLL:
         L.vo_fcc.restype = d
         L.vo_random_insertion.argtypes = [sz, dp, d, d, u64, dp, dp]
         L.vo_random_insertion.restype = i32
-        L.vo_derive_params.argtypes = [dp, sz, dp, d, d, d, i32, i32, i32, d, d, u64, dp, C.POINTER(Params)]
+        L.vo_derive_params.argtypes = [dp, sz, dp, d, d, d, i32, i32, i32, d, d, d, u64, dp, C.POINTER(Params)]
         L.vo_derive_params.restype = i32
-        L.vo_create.argtypes = [dp, dp, sz, dp, d, d, d, i32, i32, i32, d, d, u64]
+        L.vo_create.argtypes = [dp, dp, sz, dp, d, d, d, i32, i32, i32, d, d, d, u64]
+        L.vo_set_temperature.argtypes = [P, d]
         L.vo_create.restype = P
         L.vo_destroy.argtypes = [P]
         L.vo_get_params.argtypes = [P, C.POINTER(Params)]
@@ -208,13 +213,13 @@ def all_pairs(pos: np.ndarray, box, cutoff: float, shifted_force: bool, forces: 
 
 
 def derive_params(pos, vel, box, cutoff=2.5, skin=0.3, dt=0.005, shifted_force=False, precision=0,
-                  thermostat=0, temperature=0.0, friction=0.0, seed=1) -> Params:
+                  thermostat=0, temperature=0.0, friction=0.0, seed=1, period=0.0) -> Params:
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     vel = np.ascontiguousarray(vel, dtype=np.float64)
     b = np.ascontiguousarray(box, dtype=np.float64)
     p = Params()
     rc = lib().vo_derive_params(_dp(pos), pos.shape[0], _dp(b), cutoff, skin, dt, int(shifted_force), precision,
-                                thermostat, temperature, friction, seed, _dp(vel), C.byref(p))
+                                thermostat, temperature, friction, period, seed, _dp(vel), C.byref(p))
     if rc != 0:
         raise ValueError("the box must be at least two (cutoff + skin) wide on every axis")
     return p
@@ -256,13 +261,13 @@ class Sim:
     """The oracle's simulation, mirroring the product's step/measure API."""
 
     def __init__(self, pos, vel, box, cutoff=2.5, skin=0.3, dt=0.005, shifted_force=False, precision=0,
-                 thermostat=0, temperature=0.0, friction=0.0, seed=1):
+                 thermostat=0, temperature=0.0, friction=0.0, seed=1, period=0.0):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         vel = np.ascontiguousarray(vel, dtype=np.float64)
         b = np.ascontiguousarray(box, dtype=np.float64)
         self.n = pos.shape[0]
         self._h = lib().vo_create(_dp(pos), _dp(vel), self.n, _dp(b), cutoff, skin, dt, int(shifted_force),
-                                  precision, thermostat, temperature, friction, seed)
+                                  precision, thermostat, temperature, friction, period, seed)
         if not self._h:
             raise ValueError("the box must be at least two (cutoff + skin) wide on every axis")
         self.params = Params()
@@ -275,6 +280,10 @@ class Sim:
 
     def __del__(self):
         self.close()
+
+    def set_temperature(self, t: float):
+        lib().vo_set_temperature(self._h, t)
+        lib().vo_get_params(self._h, C.byref(self.params))
 
     def set_near_count(self, k: int):
         lib().vo_set_near_count(self._h, k)

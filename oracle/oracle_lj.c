@@ -28,12 +28,15 @@
  *     keeps its image s.
  *  R5 pair, FP64, over the list in order with d = (xj + s L) - xi and the
  *     entry's image s: r2 < rc^2; inv2 = 1 / r2; s6 = inv2 inv2 inv2;
- *     rf = s6 fma(48, s6, -24); u = fma(4 s6, s6, -4 s6) - shift; shifted
- *     force: r = sqrt(r2), rf = fma(-r, fs, rf), u = fma(r - rc, fs, u);
- *     fscal = rf inv2; a -= d fscal as fma(-d, fscal, a) (lennard_jones.wgsl:501-545).
+ *     g = s6 fma(s6, 2, -1) (that is r.F / 24); shifted force:
+ *     g = fma(-sqrt(r2), fs / 24, g); fscal = g inv2; G = fma(d, fscal, G)
+ *     over the list, and a = -24 G at the end (lennard_jones.wgsl:501-545,
+ *     with the factor 24 of r.F = 24 s6 (2 s6 - 1) taken out of the sum).
  *  R5f pair, FP32 forces: cutoff test and d as R5 in FP64, then in f32:
- *     q = (float)r2, inv2 = 1/q, s6, rf, fscal as R5; (float)d * fscal
- *     accumulated in FP64.
+ *     q = (float)r2, inv2 = 1/q, s6, g, fscal as R5; (float)d * fscal
+ *     accumulated in FP64, a = -24 G.
+ *  R9 energies: rf = s6 fma(48, s6, -24); u = fma(4 s6, s6, -4 s6) - shift;
+ *     shifted force: r = sqrt(r2), rf = fma(-r, fs, rf), u = fma(r - rc, fs, u).
  *  R6 drift (velocity Verlet): v = fma(a, dt/2, v); x = fma(v, dt, x);
  *     moved^2 = r2 of x - x_build (lennard_jones.wgsl:204-276).
  *  R7 kick: v = fma(a, dt/2, v) (lennard_jones.wgsl:555-574).
@@ -59,6 +62,7 @@
 #endif
 
 #define TWO32 4294967296.0
+#define VO_PI 3.14159265358979323846 /* the f64 nearest pi (std::f64::consts::PI) */
 
 struct vo_sim {
     vo_params p;
@@ -75,6 +79,7 @@ struct vo_sim {
     int64_t *prof;
     double max_moved2;
     uint64_t steps;
+    double xi, eta, nh_sm1; /* Nose-Hoover friction, its integral, last scale - 1 */
 };
 
 /* ------------------------------------------------------------------ */
@@ -488,8 +493,8 @@ static inline uint32_t cell_of_position(const vo_params *p, double x, double y, 
  *  - dt stays FP64 (the reference rounds it to f32 for its shaders). */
 int vo_derive_params(const double *pos, size_t n, const double box_in[3], double cutoff,
                      double skin, double dt, int shifted_force, int precision,
-                     int thermostat, double temperature, double friction, uint64_t seed,
-                     const double *vel, vo_params *p)
+                     int thermostat, double temperature, double friction, double period,
+                     uint64_t seed, const double *vel, vo_params *p)
 {
     memset(p, 0, sizeof(*p));
     double list_radius = cutoff + skin;
@@ -550,7 +555,7 @@ int vo_derive_params(const double *pos, size_t n, const double box_in[3], double
         kinetic += 0.5 * v2;
         speed += sqrt(v2);
     }
-    double thermal = thermostat == 1 ? 1.5 * temperature * (double)n : 0.0;
+    double thermal = thermostat != 0 ? 1.5 * temperature * (double)n : 0.0;
     double m = kinetic > thermal ? kinetic : thermal;
     if ((double)n > m)
         m = (double)n;
@@ -572,6 +577,14 @@ int vo_derive_params(const double *pos, size_t n, const double box_in[3], double
     p->thermostat = thermostat;
     p->c1 = 1.0;
     p->c2 = 0.0;
+    p->nh_q = 1.0;
+    p->nh_period = period;
+    if (thermostat == 2) {
+        /* lennard_jones.rs:516-528: zero total momentum, g = 3N - 3 */
+        double dof = 3.0 * (double)n - 3.0;
+        p->nh_gkt = dof * temperature;
+        p->nh_q = dof * temperature * period * period / (4.0 * (VO_PI * VO_PI));
+    }
     if (thermostat == 1) {
         p->c1 = exp(-friction * dt);
         p->c2 = sqrt((1.0 - p->c1 * p->c1) * temperature);
@@ -596,11 +609,12 @@ static void *xcalloc(size_t n, size_t size)
 
 vo_sim *vo_create(const double *pos, const double *vel, size_t n, const double box[3],
                   double cutoff, double skin, double dt, int shifted_force, int precision,
-                  int thermostat, double temperature, double friction, uint64_t seed)
+                  int thermostat, double temperature, double friction, double period,
+                  uint64_t seed)
 {
     vo_sim *s = (vo_sim *)xcalloc(1, sizeof(vo_sim));
     if (vo_derive_params(pos, n, box, cutoff, skin, dt, shifted_force, precision, thermostat,
-                         temperature, friction, seed, vel, &s->p) != 0) {
+                         temperature, friction, period, seed, vel, &s->p) != 0) {
         free(s);
         return NULL;
     }
@@ -796,12 +810,14 @@ static void update_lists(vo_sim *s)
     build_lists(s);
 }
 
-/* R5 / R5f, lennard_jones.wgsl:501-545. energies: also pe, w, near, within. */
+/* R5 / R5f, lennard_jones.wgsl:501-545: the acceleration of atom i; with
+ * `energies` also its pe, w, near and within for the measurement (R9). */
 static void interactions(vo_sim *s, int32_t i, double acc[3], int energies)
 {
     const vo_params *p = &s->p;
+    const double fs24 = p->force_shift / 24.0;
     double xi = s->x[0][i], yi = s->x[1][i], zi = s->x[2][i];
-    double ax = 0.0, ay = 0.0, az = 0.0, pe = 0.0, w = 0.0;
+    double gx = 0.0, gy = 0.0, gz = 0.0, pe = 0.0, w = 0.0;
     uint32_t near = 0, within = 0;
     uint32_t cnt = s->nbr_count[i];
     const uint32_t *row = s->nbr + (size_t)i * p->capacity;
@@ -818,41 +834,51 @@ static void interactions(vo_sim *s, int32_t i, double acc[3], int energies)
         double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (!(r2 < p->rc2))
             continue;
-        double rf, u;
+        double rf = 0.0, u = 0.0;
         if (p->precision == 0) {
             double inv2 = 1.0 / r2;
             double s6 = inv2 * inv2 * inv2;
-            rf = s6 * fma(48.0, s6, -24.0);
-            double t = 4.0 * s6;
-            u = fma(t, s6, -t) - p->shift;
-            if (p->shifted_force) {
-                double r = sqrt(r2);
-                rf = fma(-r, p->force_shift, rf);
-                u = fma(r - p->rc, p->force_shift, u);
+            double g = s6 * fma(s6, 2.0, -1.0); /* r.F / 24 */
+            if (p->shifted_force)
+                g = fma(-sqrt(r2), fs24, g);
+            double fscal = g * inv2;
+            gx = fma(dx, fscal, gx);
+            gy = fma(dy, fscal, gy);
+            gz = fma(dz, fscal, gz);
+            if (energies) {
+                rf = s6 * fma(48.0, s6, -24.0);
+                double t = 4.0 * s6;
+                u = fma(t, s6, -t) - p->shift;
+                if (p->shifted_force) {
+                    double r = sqrt(r2);
+                    rf = fma(-r, p->force_shift, rf);
+                    u = fma(r - p->rc, p->force_shift, u);
+                }
             }
-            double fscal = rf * inv2;
-            ax = fma(-dx, fscal, ax);
-            ay = fma(-dy, fscal, ay);
-            az = fma(-dz, fscal, az);
         } else {
             float q = (float)r2;
             float inv2 = 1.0f / q;
             float s6 = inv2 * inv2 * inv2;
-            float rff = s6 * fmaf(48.0f, s6, -24.0f);
-            float t = 4.0f * s6;
-            float uf = fmaf(t, s6, -t) - (float)p->shift;
-            if (p->shifted_force) {
-                float r = sqrtf(q);
-                float fs = (float)p->force_shift;
-                rff = fmaf(-r, fs, rff);
-                uf = fmaf(r - (float)p->rc, fs, uf);
+            float g = s6 * fmaf(s6, 2.0f, -1.0f);
+            if (p->shifted_force)
+                g = fmaf(-sqrtf(q), (float)fs24, g);
+            float fscal = g * inv2;
+            gx += (double)((float)dx * fscal);
+            gy += (double)((float)dy * fscal);
+            gz += (double)((float)dz * fscal);
+            if (energies) {
+                float rff = s6 * fmaf(48.0f, s6, -24.0f);
+                float t = 4.0f * s6;
+                float uf = fmaf(t, s6, -t) - (float)p->shift;
+                if (p->shifted_force) {
+                    float r = sqrtf(q);
+                    float fs = (float)p->force_shift;
+                    rff = fmaf(-r, fs, rff);
+                    uf = fmaf(r - (float)p->rc, fs, uf);
+                }
+                rf = (double)rff;
+                u = (double)uf;
             }
-            float fscal = rff * inv2;
-            ax -= (double)((float)dx * fscal);
-            ay -= (double)((float)dy * fscal);
-            az -= (double)((float)dz * fscal);
-            rf = (double)rff;
-            u = (double)uf;
         }
         if (energies) {
             pe += 0.5 * u;
@@ -861,9 +887,9 @@ static void interactions(vo_sim *s, int32_t i, double acc[3], int energies)
             within += 1u;
         }
     }
-    acc[0] = ax;
-    acc[1] = ay;
-    acc[2] = az;
+    acc[0] = -24.0 * gx;
+    acc[1] = -24.0 * gy;
+    acc[2] = -24.0 * gz;
     if (energies) {
         s->pe[i] = pe;
         s->w[i] = w;
@@ -912,7 +938,10 @@ static void drift(vo_sim *s)
             vo_langevin_noise(p->seed, 0u, s->orig[i], step, noise);
         double x[3];
         for (int k = 0; k < 3; k++) {
-            double v = fma(s->a[k][i], p->half_dt, s->v[k][i]);
+            double v0 = s->v[k][i];
+            if (p->thermostat == 2) /* R13: the first half step's scaling */
+                v0 = fma(v0, s->nh_sm1, v0);
+            double v = fma(s->a[k][i], p->half_dt, v0);
             double xk = s->x[k][i];
             if (p->thermostat == 1) {
                 xk = fma(v, p->half_dt, xk);
@@ -935,15 +964,69 @@ static void drift(vo_sim *s)
     s->max_moved2 = maxd;
 }
 
-/* lennard_jones.rs:1092-1138 (NVE and Langevin) */
+/* R13 Nose-Hoover half step (lennard_jones.wgsl:309-374): KE from the
+ * fixed-point sum, then xi, eta and the velocity scale s - 1 = exp(-xi dt/2) - 1
+ * by its series, in FP64. */
+static void nh_half(vo_sim *s)
+{
+    const vo_params *p = &s->p;
+    int32_t n = (int32_t)p->count;
+    int64_t raw = 0;
+#pragma omp parallel for schedule(static) reduction(+ : raw)
+    for (int32_t i = 0; i < n; i++) {
+        double vx = s->v[0][i], vy = s->v[1][i], vz = s->v[2][i];
+        raw += vo_fixed_f64(0.5 * fma(vz, vz, fma(vy, vy, vx * vx)), p->energy_fixed);
+    }
+    double ke = (double)raw / p->energy_fixed;
+    double quarter = 0.25 * p->dt;
+    s->xi = fma((2.0 * ke - p->nh_gkt) / p->nh_q, quarter, s->xi);
+    double x = s->xi * p->half_dt;
+    double sm1 = -x * (1.0 - x * (0.5 - x * (1.0 / 6.0 - x / 24.0)));
+    s->eta = fma(s->xi, p->half_dt, s->eta);
+    double ke2 = fma(ke, sm1 * (2.0 + sm1), ke);
+    s->xi = fma((2.0 * ke2 - p->nh_gkt) / p->nh_q, quarter, s->xi);
+    s->nh_sm1 = sm1;
+}
+
+static void nh_scale(vo_sim *s)
+{
+    int32_t n = (int32_t)s->p.count;
+    double sm1 = s->nh_sm1;
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++)
+            s->v[k][i] = fma(s->v[k][i], sm1, s->v[k][i]);
+}
+
+/* lennard_jones.rs:1092-1138 (NVE, Langevin, Nose-Hoover) */
 void vo_run(vo_sim *s, uint64_t steps)
 {
     vo_prime(s);
+    int nh = s->p.thermostat == 2;
     for (uint64_t t = 0; t < steps; t++) {
+        if (nh)
+            nh_half(s);
         drift(s);
         update_lists(s);
         compute_forces(s, 1);
+        if (nh) {
+            nh_half(s);
+            nh_scale(s);
+        }
         s->steps++;
+    }
+}
+
+/* lennard_jones.rs:905-925 */
+void vo_set_temperature(vo_sim *s, double temperature)
+{
+    vo_params *p = &s->p;
+    if (p->thermostat == 1) {
+        p->c2 = sqrt((1.0 - p->c1 * p->c1) * temperature);
+    } else if (p->thermostat == 2) {
+        double dof = 3.0 * (double)p->count - 3.0;
+        p->nh_gkt = dof * temperature;
+        p->nh_q = p->nh_gkt * p->nh_period * p->nh_period / (4.0 * (VO_PI * VO_PI));
     }
 }
 
@@ -1115,6 +1198,7 @@ void vo_measure(vo_sim *s, vo_measurement *m)
     m->count = p->count;
     m->volume = p->box[0] * p->box[1] * p->box[2];
     m->list_rebuilds = s->rebuilds;
+    m->thermostat_energy = p->thermostat == 2 ? 0.5 * p->nh_q * s->xi * s->xi + p->nh_gkt * s->eta : 0.0;
     m->list_overflow = s->overflow;
     m->condensed = (uint32_t)cond;
     m->pairs_within_cutoff = (uint64_t)within / 2u;

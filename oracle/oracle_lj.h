@@ -46,6 +46,8 @@ typedef struct {
     int32_t precision;      /* 0 FP64 pair forces, 1 FP32 pair forces */
     int32_t shifted_force;
     uint32_t near_count;    /* condensed criterion (rs:557) */
+    double nh_gkt, nh_q;    /* Nose-Hoover g k T and mass Q (rs:516-528) */
+    double nh_period;
 } vo_params;
 
 typedef struct {
@@ -57,6 +59,7 @@ typedef struct {
     uint32_t condensed;
     uint64_t pairs_within_cutoff;
     int64_t raw[8];         /* fixed-point sums: KE, PE, W, Px, Py, Pz, condensed, 2*pairs */
+    double thermostat_energy; /* Nose-Hoover: Q xi^2 / 2 + g k T eta (rs:1235-1241) */
 } vo_measurement;
 
 typedef struct vo_sim vo_sim;
@@ -82,14 +85,18 @@ int vo_random_insertion(size_t count, const double box[3], double min_distance,
                         double temperature, uint64_t seed, double *pos, double *vel);
 
 /* ---- simulation ---- */
-/* thermostat: 0 none, 1 Langevin. precision: 0 FP64, 1 FP32 pair forces. */
+/* thermostat: 0 none, 1 Langevin (friction, seed), 2 Nose-Hoover (period).
+ * precision: 0 FP64, 1 FP32 pair forces. */
 int vo_derive_params(const double *pos, size_t n, const double box[3], double cutoff,
                      double skin, double dt, int shifted_force, int precision,
-                     int thermostat, double temperature, double friction, uint64_t seed,
-                     const double *vel, vo_params *out);
+                     int thermostat, double temperature, double friction, double period,
+                     uint64_t seed, const double *vel, vo_params *out);
 vo_sim *vo_create(const double *pos, const double *vel, size_t n, const double box[3],
                   double cutoff, double skin, double dt, int shifted_force, int precision,
-                  int thermostat, double temperature, double friction, uint64_t seed);
+                  int thermostat, double temperature, double friction, double period,
+                  uint64_t seed);
+/* set_temperature (lennard_jones.rs:905-925); no effect without a thermostat. */
+void vo_set_temperature(vo_sim *s, double temperature);
 void vo_destroy(vo_sim *s);
 void vo_get_params(const vo_sim *s, vo_params *out);
 void vo_set_near_count(vo_sim *s, uint32_t near_count);

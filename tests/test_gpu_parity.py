@@ -52,6 +52,26 @@ def test_fixed_point_sum_kat(lj):
     assert runs == [expected] * 3
 
 
+def test_force_reciprocal_rounds_like_division(lj):
+    # the force loop's 1 / r^2 must be the correctly rounded quotient that
+    # the oracle's 1.0 / r2 gives, over the whole range r^2 takes
+    rng = np.random.default_rng(5)
+    x = np.concatenate([
+        10.0 ** rng.uniform(-4, 37, 4_000_000),
+        rng.uniform(0.5, 8.0, 2_000_000),                       # liquid pair distances
+        2.0 ** np.arange(-13.0, 122.0),                          # exact powers of two
+        np.nextafter(2.0 ** np.arange(-13.0, 122.0), np.inf),
+        np.nextafter(2.0 ** np.arange(-13.0, 122.0), 0.0),
+        1.0 + np.arange(1, 20000) * 2.0 ** -52,                  # just above 1
+        1.0 - np.arange(1, 20000) * 2.0 ** -53,                  # just below 1
+        np.array([3e36, 6.25, 12.25, 7.84, 1e-4]),
+    ])
+    got = lj.debug_rcp(x)
+    want = 1.0 / x
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, f"{bad.size} differ, first x = {x[bad[:5]]!r}"
+
+
 def test_cell_grid_finds_every_pair(oracle, lj):
     # tests/thermodynamics.rs:55-98, at the reference's tolerance and bit for bit
     pos, vel = jittered_lattice(oracle)

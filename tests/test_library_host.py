@@ -34,9 +34,9 @@ def test_create_validates_before_touching_a_device(lj):
     with pytest.raises(lj.LennardJonesError, match="at least two \\(cutoff \\+ skin\\) wide") as e:
         lj.LennardJones((pos, pos), lj.Settings([5.5, 20.0, 20.0], 2.5, 0.005))
     assert e.value.code == 1
-    with pytest.raises(lj.LennardJonesError, match="Nose-Hoover") as e:
-        lj.LennardJones((pos, pos), lj.Settings([20.0] * 3, 2.5, 0.005, thermostat=lj.NoseHoover(1.0, 0.5)))
-    assert e.value.code == 4
+    with pytest.raises(lj.LennardJonesError, match="Nose-Hoover needs .* positive period") as e:
+        lj.LennardJones((pos, pos), lj.Settings([20.0] * 3, 2.5, 0.005, thermostat=lj.NoseHoover(1.0, 0.0)))
+    assert e.value.code == 1
     with pytest.raises(lj.LennardJonesError, match="finite"):
         bad = pos.copy()
         bad[0, 0] = np.nan
@@ -80,6 +80,45 @@ def test_host_lattices_match_the_oracle(oracle, lj):
     f, side = lj.fcc_array(5, 0.8442, 0.722, 7)
     p, v, s = oracle.fcc(5, 0.8442, 0.722, 7)
     assert np.array_equal(f["pos"], p) and np.array_equal(f["vel"], v) and side == s
+
+
+def test_random_insertion_matches_the_oracle(oracle, lj):
+    # the library's grid-accelerated insertion takes the same decisions as
+    # the oracle's check against every earlier particle
+    box = [12.0, 9.5, 14.0]
+    a = lj.random_insertion_array(400, box, 0.9, 1.3, 11)
+    p, v = oracle.random_insertion(400, box, 0.9, 1.3, 11)
+    assert np.array_equal(a["pos"], p) and np.array_equal(a["vel"], v)
+    pa = np.ascontiguousarray(a["pos"])
+    d = pa[:, None, :] - pa[None, :, :]
+    d -= np.array(box) * np.round(d / np.array(box))
+    r = np.sqrt((d ** 2).sum(-1)) + np.eye(400) * 1e9
+    assert r.min() >= 0.9
+    assert np.abs(a["vel"].sum(axis=0)).max() < 1e-9
+    with pytest.raises(lj.LennardJonesError, match="no free place"):
+        lj.random_insertion_array(200, [3.0, 3.0, 3.0], 0.9, 1.0, 1)
+
+
+@pytest.mark.parametrize("count", [1, 500, 4 * 5 ** 3, 4 * 5 ** 3 + 1, 1201])
+def test_fcc_for_any_count(lj, count):
+    f, side = lj.fcc_count_array(count, 0.8442, 1.0, 5)
+    assert side == float(np.float32((count / 0.8442) ** (1 / 3)))
+    assert f.shape[0] == count
+    assert (f["pos"] >= 0).all() and (f["pos"] < side).all()
+    n = 1
+    while 4 * n ** 3 < count:
+        n += 1
+    if count > 1:
+        pf = np.ascontiguousarray(f["pos"])
+        d = pf[:, None, :] - pf[None, :, :]
+        d -= side * np.round(d / side)
+        r = np.sqrt((d ** 2).sum(-1)) + np.eye(count) * 1e9
+        assert r.min() == pytest.approx(side / n / np.sqrt(2), rel=1e-12)
+        assert np.abs(f["vel"].sum(axis=0)).max() < 1e-9
+    if count == 4 * 5 ** 3:  # a full lattice: the same sites as fcc(5), up to the side
+        g, gside = lj.fcc_array(5, 0.8442, 1.0, 5)
+        np.testing.assert_allclose(f["pos"] / side, g["pos"] / gside, rtol=1e-12, atol=1e-15)
+        assert np.array_equal(f["vel"], g["vel"])
 
 
 def test_lattice_api_mirror(lj):

@@ -187,3 +187,81 @@ def test_langevin_holds_the_temperature(oracle):
         temps.append(2 * s.measure().kinetic / (3 * n))
     mean = np.mean(temps)
     assert abs(mean / T - 1) < 0.03
+
+
+def nh_sim(oracle, n=512, T=1.5, start_T=0.5, period=0.5, seed=21):
+    box = (n / 0.3) ** (1 / 3)
+    pos, vel = oracle.lattice(n, box, start_T, seed)
+    return oracle.Sim(pos, vel, [box] * 3, dt=0.002, shifted_force=True, thermostat=2, temperature=T,
+                      period=period), n
+
+
+def test_nose_hoover_parameters(oracle):
+    # lennard_jones.rs:516-528: g = 3N - 3, Q = g k T period^2 / (4 pi^2);
+    # set_temperature keeps the period (rs:912-920)
+    s, n = nh_sim(oracle)
+    g = 3 * n - 3
+    assert s.params.nh_gkt == g * 1.5
+    assert s.params.nh_q == pytest.approx(g * 1.5 * 0.25 / (4 * np.pi ** 2), rel=1e-15)
+    s.set_temperature(0.75)
+    assert s.params.nh_gkt == g * 0.75
+    assert s.params.nh_q == pytest.approx(g * 0.75 * 0.25 / (4 * np.pi ** 2), rel=1e-15)
+
+
+def test_nose_hoover_conserves_the_extended_energy(oracle):
+    # KE + PE + Q xi^2/2 + g k T eta is the conserved quantity of the
+    # splitting: its largest deviation over the same 4 time units falls as
+    # dt^2 (a wrong sign or a missing term would leave an O(1) drift), and
+    # the thermostat heats a cold start to the target on average
+    n = 512
+    box = (n / 0.3) ** (1 / 3)
+    pos, vel = oracle.lattice(n, box, 0.5, 21)
+    s = oracle.Sim(pos, vel, [box] * 3, dt=0.002, shifted_force=True)
+    s.run(1000)
+    p, v = s.read_particles()
+    devs, temps = [], []
+    for dt in (0.004, 0.002):
+        t = oracle.Sim(p, v, [box] * 3, dt=dt, shifted_force=True, thermostat=2, temperature=1.5, period=0.5)
+        m0 = t.measure()
+        e0 = m0.kinetic + m0.potential + m0.thermostat_energy
+        dev = 0.0
+        every = int(round(0.2 / dt))
+        for _ in range(20):
+            t.run(every)
+            m = t.measure()
+            dev = max(dev, abs(m.kinetic + m.potential + m.thermostat_energy - e0))
+            temps.append(2 * m.kinetic / (3 * n - 3))
+        devs.append(dev)
+    assert 3.0 < devs[0] / devs[1] < 5.0
+    assert abs(np.mean(temps) / 1.5 - 1) < 0.15
+
+
+def test_nose_hoover_half_step_closed_form(oracle):
+    # with no pair inside the cutoff the velocities only feel the thermostat:
+    # two particles far apart, KE fixed by hand; one step scales v by
+    # s1 = exp(-xi1 dt/2) twice, xi following the rule of R13
+    box = [12.0] * 3
+    pos = np.array([[1.0, 1.0, 1.0], [7.0, 7.0, 7.0]])
+    vel = np.array([[0.3, -0.2, 0.1], [-0.3, 0.2, -0.1]])
+    T, period, dt = 1.0, 0.5, 0.002
+    s = oracle.Sim(pos, vel, box, dt=dt, thermostat=2, temperature=T, period=period)
+    g = 3 * 2 - 3
+    gkt, Q = g * T, g * T * period ** 2 / (4 * np.pi ** 2)
+    ke = 0.5 * np.sum(vel ** 2)
+    xi = eta = 0.0
+
+    def half(ke, xi, eta):
+        xi += (2 * ke - gkt) / Q * dt / 4
+        s = np.exp(-xi * dt / 2)
+        eta += xi * dt / 2
+        xi += (2 * ke * s * s - gkt) / Q * dt / 4
+        return s, xi, eta
+
+    s1, xi, eta = half(ke, xi, eta)
+    ke *= s1 * s1
+    s2, xi, eta = half(ke, xi, eta)
+    s.run(1)
+    _, v = s.read_particles()
+    np.testing.assert_allclose(v, vel * s1 * s2, rtol=1e-12)
+    m = s.measure()
+    assert m.thermostat_energy == pytest.approx(0.5 * Q * xi * xi + gkt * eta, rel=1e-9)

@@ -42,15 +42,16 @@ struct Replica {
   uint32_t scan_block_off, scan_blocks;
   uint32_t atom_block_off, atom_blocks;  // this replica's entries of the atom-block table
   uint32_t capacity;
-  uint32_t thermostat;  // 0 none, 1 Langevin (BAOAB)
+  uint32_t thermostat;  // 0 none, 1 Langevin (BAOAB), 2 Nose-Hoover
   uint64_t list_off;    // first neighbour entry of the replica
   double rc, rc2, list_r2, half_skin2;
-  double shift, force_shift;
+  double shift, force_shift, force_shift24;  // force_shift24 = force_shift / 24
   double dt, half_dt, c1, c2;
   double energy_fixed, momentum_fixed;  // scale * 2^32
   uint64_t seed;
   uint32_t near_count;
   uint32_t index;
+  double nh_gkt, nh_q, nh_period;  // Nose-Hoover g k T, mass Q and period (thermostat 2)
   double bin_width;    // profile slab height, L_z / prof_bins
   uint32_t prof_bins;  // profile slabs along z (0: no profiles)
   uint32_t prof_off;   // first of the replica's prof_bins * 8 + 3 profile sums
@@ -67,6 +68,7 @@ struct Control {
   uint32_t overflow;
   uint32_t built;
   uint32_t pad;
+  double xi, eta, sm1;            // Nose-Hoover friction, its integral, velocity scale - 1
 };
 
 struct Range {
@@ -90,6 +92,7 @@ struct Buffers {
   const Brick* bricks;        // list build, forces and measurements
   uint32_t n_atom_blocks, n_scan_blocks, n_sort_blocks, n_bricks, n_replicas;
   unsigned long long* bmax;   // per atom block: max moved^2 (FP64 bits)
+  long long* kepart;          // per atom block: kinetic energy in fixed point (Nose-Hoover)
   double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;           // state, slot order
   double *x2, *y2, *z2, *vx2, *vy2, *vz2, *ax2, *ay2, *az2;  // reorder target
   double *xb, *yb, *zb;                                      // position at last list build

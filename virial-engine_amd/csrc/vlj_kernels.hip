@@ -127,9 +127,13 @@ __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
     const double* A[3] = {B.ax, B.ay, B.az};
     const double* XB[3] = {B.xb, B.yb, B.zb};
     double d[3];
+    const bool nh = R.thermostat == 2u;
+    const double sm1 = nh ? B.ctl[blk.rep].sm1 : 0.0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      double v = __builtin_fma(A[k][i], R.half_dt, V[k][i]);
+      double v0 = V[k][i];
+      if (nh) v0 = __builtin_fma(v0, sm1, v0);  // R13: the first half step's scaling
+      double v = __builtin_fma(A[k][i], R.half_dt, v0);
       double x = X[k][i];
       if (baoab) {
         x = __builtin_fma(v, R.half_dt, x);
@@ -159,24 +163,113 @@ __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
 }
 
 // ---------------------------------------------------------------------------
+// Nose-Hoover half step (lennard_jones.wgsl:309-397, R13): the kinetic
+// energy in fixed point per atom block, then per replica xi, eta and the
+// velocity scale s - 1 = exp(-xi dt / 2) - 1 by its series, in FP64; the
+// second half step's scaling after the force kick.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_ke_sum(Buffers B) {
+  const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const Range blk = B.atom_blocks[b];
+  const Replica R = B.rep[blk.rep];
+  if (R.thermostat != 2u) return;
+  const uint32_t i = blk.first + threadIdx.x;
+  long long ke = 0;
+  if (i < blk.end) ke = to_fixed(0.5 * dist2(B.vx[i], B.vy[i], B.vz[i]), R.energy_fixed);
+  __shared__ long long part[kBlock / 64];
+  ke = wave_sum_i64(ke);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = ke;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) t += part[w];
+    B.kepart[b] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_nh_update(Buffers B) {
+  const uint32_t r = blockIdx.x;
+  const Replica R = B.rep[r];
+  if (R.thermostat != 2u) return;
+  long long t = 0;
+  for (uint32_t b = R.atom_block_off + threadIdx.x; b < R.atom_block_off + R.atom_blocks; b += kBlock)
+    t += B.kepart[b];
+  __shared__ long long part[kBlock / 64];
+  t = wave_sum_i64(t);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long raw = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) raw += part[w];
+    Control* c = B.ctl + r;
+    const double ke = (double)raw / R.energy_fixed;
+    const double quarter = 0.25 * R.dt;
+    double xi = __builtin_fma((2.0 * ke - R.nh_gkt) / R.nh_q, quarter, c->xi);
+    const double x = xi * R.half_dt;
+    const double sm1 = -x * (1.0 - x * (0.5 - x * (1.0 / 6.0 - x / 24.0)));
+    c->eta = __builtin_fma(xi, R.half_dt, c->eta);
+    const double ke2 = __builtin_fma(ke, sm1 * (2.0 + sm1), ke);
+    c->xi = __builtin_fma((2.0 * ke2 - R.nh_gkt) / R.nh_q, quarter, xi);
+    c->sm1 = sm1;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_nh_scale(Buffers B) {
+  const Range blk = B.atom_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const Replica R = B.rep[blk.rep];
+  if (R.thermostat != 2u) return;
+  const uint32_t i = blk.first + threadIdx.x;
+  if (i >= blk.end) return;
+  const double sm1 = B.ctl[blk.rep].sm1;
+  B.vx[i] = __builtin_fma(B.vx[i], sm1, B.vx[i]);
+  B.vy[i] = __builtin_fma(B.vy[i], sm1, B.vy[i]);
+  B.vz[i] = __builtin_fma(B.vz[i], sm1, B.vz[i]);
+}
+
+// ---------------------------------------------------------------------------
 // Cell grid (grid.wgsl:22-148) with a deterministic order: counts, scan,
 // scatter in any order, then a rank sort inside each cell by slot index.
 // ---------------------------------------------------------------------------
+
+// Adds each active lane's 1 to counter[key] and returns the counter's value
+// before this lane's add. Lanes with the same key side by side (atoms are
+// in cell order after every rebuild, so runs of ~20 are the rule) share one
+// atomic: the run's first lane adds the run length for all of them.
+__device__ __forceinline__ uint32_t run_atomic_add(uint32_t* counter, uint32_t key, bool active) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t k = active ? key : 0xffffffffu;
+  const uint32_t prev = __shfl_up(k, 1, 64);
+  const bool head = lane == 0u || prev != k;
+  const unsigned long long heads = __builtin_amdgcn_ballot_w64(head);
+  const unsigned long long upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);  // lanes <= this one
+  const uint32_t leader = 63u - (uint32_t)__builtin_clzll(heads & upto);
+  const unsigned long long above = heads & ~upto;
+  const uint32_t next = above ? (uint32_t)__builtin_ctzll(above) : 64u;
+  uint32_t base = 0;
+  if (head && active) base = atomicAdd(counter + key, next - lane);
+  return (uint32_t)__shfl(base, (int)leader, 64) + (lane - leader);
+}
 
 __global__ void __launch_bounds__(kBlock) k_bin(Buffers B) {
   const Range blk = B.atom_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
   if (!B.ctl[blk.rep].rebuild) return;
   const Replica R = B.rep[blk.rep];
   const uint32_t i = blk.first + threadIdx.x;
-  if (i >= blk.end) return;
-  // R1: positions go back into the box when the lists are rebuilt
-  const double x = wrap_one(B.x[i], R.box[0]), y = wrap_one(B.y[i], R.box[1]), z = wrap_one(B.z[i], R.box[2]);
-  B.x[i] = x;
-  B.y[i] = y;
-  B.z[i] = z;
-  uint32_t c = R.cell_off + cell_local(R, x, y, z);
-  B.cell_of[i] = c;
-  atomicAdd(B.cell_count + c, 1u);
+  const bool active = i < blk.end;
+  uint32_t c = 0;
+  if (active) {
+    // R1: positions go back into the box when the lists are rebuilt
+    const double x = wrap_one(B.x[i], R.box[0]), y = wrap_one(B.y[i], R.box[1]), z = wrap_one(B.z[i], R.box[2]);
+    B.x[i] = x;
+    B.y[i] = y;
+    B.z[i] = z;
+    c = R.cell_off + cell_local(R, x, y, z);
+    B.cell_of[i] = c;
+  }
+  run_atomic_add(B.cell_count, c, active);
 }
 
 // Exclusive scan of counts inside each block of 1024 cells (4 per thread);
@@ -239,10 +332,10 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
   if (!B.ctl[blk.rep].rebuild) return;
   const Replica R = B.rep[blk.rep];
   const uint32_t i = blk.first + threadIdx.x;
-  if (i >= blk.end) return;
-  uint32_t c = B.cell_of[i];
-  uint32_t slot = cell_first_slot(B, R, c) + atomicAdd(B.cell_cursor + c, 1u);
-  B.unsorted[slot] = i;
+  const bool active = i < blk.end;
+  const uint32_t c = active ? B.cell_of[i] : 0u;
+  const uint32_t rank = run_atomic_add(B.cell_cursor, c, active);
+  if (active) B.unsorted[cell_first_slot(B, R, c) + rank] = i;
 }
 
 // One wave64 per cell: every entry's rank is the number of smaller entries
@@ -472,41 +565,27 @@ __device__ __forceinline__ void staged_pos(const Replica& R, const Stage& S, boo
 // Verlet lists (lennard_jones.wgsl:454-499)
 // ---------------------------------------------------------------------------
 
-// Appends one atom's list entries eight at a time (one 16-byte store).
-// The octet being filled is a 128-bit shift register (lo, hi): each entry
-// comes in at the top and moves the others down by 16 bits, so after eight
-// entries the first is in the low bits.
-struct ListWriter {
-  uint16_t* base;  // list_base of the atom
-  unsigned long long lo, hi;
-  uint32_t found, cap;
-  __device__ __forceinline__ void store(uint32_t first) {
-    *(u32x4*)(base + list_entry(first)) = u32x4{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
-  }
-  __device__ __forceinline__ void push(uint32_t e) {
-    if (found < cap) {
-      lo = (lo >> 16) | (hi << 48);
-      hi = (hi >> 16) | ((unsigned long long)e << 48);
-      if ((found & 7u) == 7u) store(found & ~7u);
-    }
-    found++;
-  }
-  __device__ __forceinline__ void finish() {
-    const uint32_t stored = found < cap ? found : cap;
-    if (stored & 7u) {
-      for (uint32_t k = stored & 7u; k < 8u; k++) {  // fill the partial octet with pads
-        lo = (lo >> 16) | (hi << 48);
-        hi = (hi >> 16) | ((unsigned long long)kListPad << 48);
-      }
-      store(stored & ~7u);
-    }
-  }
-};
+// A value that is the same in every lane, as a scalar.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
-// One workgroup per brick. Each home atom walks the 27 cells around its
-// own, dz outermost, slots in cell order, and lists every other atom
-// within rc + skin by its staged index. On a reorder the workgroup also
-// copies its home atoms' state back from the second set.
+// Lane rank among the set bits of a wave mask (bits of lower lanes).
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// One workgroup per brick. The home atoms are cut into one tile per wave
+// (at most 64 atoms; more tiles when the brick has more than 256), and a
+// wave builds the lists of its tile together. For each home cell of the
+// tile and each of the nine runs (dz, dy) -- the three staged cells along x
+// of a run are staged one after the other, so a run is one range of staged
+// atoms in list order -- the 64 lanes take 64 candidates of the run at once.
+// Then, home atom by home atom, a ballot marks the candidates within
+// rc + skin (not the atom itself); each hit is stored at the atom's list
+// length plus its rank in the ballot, and the length grows by the ballot's
+// count. The lengths live one per lane (lane l: atom l of the tile). So the
+// entries come out in stencil order dz, dy, dx, then slot (R4).
+// On a reorder the workgroup also copies its home atoms' state back from
+// the second set.
 __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
   const Brick g = B.bricks[xcd_swizzle(blockIdx.x, gridDim.x)];
   const Control& ctl = B.ctl[g.rep];
@@ -520,7 +599,8 @@ __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
   const bool staged = stage_brick(B, R, g, X, Y, Z, S);
   const uint32_t total = S.total;
   const uint32_t cap = R.capacity;
-  const uint32_t H = S.hoff[g.gx * g.gy];
+  const uint32_t nh = g.gx * g.gy;
+  const uint32_t H = S.hoff[nh];
   for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
     const Home a = home_atom(g, S, h);
     const uint32_t i = a.slot;
@@ -539,57 +619,77 @@ __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
       B.orig[i] = B.orig2[i];
       B.cell_of[i] = B.cell_of2[i];
     }
-    ListWriter list = {B.nbr + list_base(R, i - R.atom_off), 0ull, 0ull, 0u, cap};
-    if (total <= 0xffffu) {
-      for (uint32_t dz = 0; dz < 3u; dz++)
-        for (uint32_t dy = 0; dy < 3u; dy++)
-          for (uint32_t dx = 0; dx < 3u; dx++) {
-            const uint32_t sc = staged_cell(g, dz, a.hy + dy, a.hx + dx);
-            const uint32_t e0 = S.off[sc], e1 = S.off[sc + 1u];
-            if (staged) {
-              // 64 candidates at a time: a branch-free pass sets one bit per
-              // atom within rc + skin, then only the set bits are appended,
-              // lowest first (slot order)
-              for (uint32_t c0 = e0; c0 < e1; c0 += 64u) {
-                const uint32_t m = e1 - c0 < 64u ? e1 - c0 : 64u;
-                unsigned long long in = 0ull;
-                for (uint32_t c = 0; c < m; c++) {
-                  const uint32_t q = c0 + c;
-                  const double ddx = S.x[q] - xi, ddy = S.y[q] - yi, ddz = S.z[q] - zi;
-                  // q == a.e is the atom itself (its home copy, unshifted)
-                  const bool hit = q != a.e && dist2(ddx, ddy, ddz) < R.list_r2;
-                  in |= (unsigned long long)hit << c;
-                }
-                while (in) {
-                  const uint32_t c = (uint32_t)__builtin_ctzll(in);
-                  in &= in - 1ull;
-                  list.push(c0 + c);
-                }
-              }
-            } else {
-              double shx, shy, shz;
-              image_shift(R, S.image[sc], shx, shy, shz);
-              for (uint32_t e2 = e0; e2 < e1; e2++) {
-                if (e2 == a.e) continue;
-                const uint32_t j = S.start[sc] + (e2 - e0);
-                const double ddx = (X[j] + shx) - xi;
-                const double ddy = (Y[j] + shy) - yi;
-                const double ddz = (Z[j] + shz) - zi;
-                if (dist2(ddx, ddy, ddz) < R.list_r2) list.push(e2);
-              }
-            }
-          }
-      list.finish();
-    }
-    // Past the capacity the list keeps its first `cap` entries and the
-    // overflow is reported (lennard_jones.wgsl:494-497); a neighbourhood too
-    // large for 16-bit staged indices keeps none.
-    const uint32_t found = total <= 0xffffu ? list.found : total;
-    B.nbr_count[i] = total <= 0xffffu ? (found < cap ? found : cap) : 0u;
-    if (found > cap) atomicMax(&B.ctl[g.rep].overflow, found);
     B.xb[i] = xi;
     B.yb[i] = yi;
     B.zb[i] = zi;
+    // A neighbourhood too large for 16-bit staged indices keeps no list;
+    // the overflow is reported (lennard_jones.wgsl:494-497).
+    if (total > 0xffffu) B.nbr_count[i] = 0u;
+  }
+  if (total > 0xffffu) {
+    if (threadIdx.x == 0 && H > 0u && total > cap) atomicMax(&B.ctl[g.rep].overflow, total);
+    return;
+  }
+  const uint32_t lane = threadIdx.x & 63u, wave = uniform(threadIdx.x >> 6);
+  uint32_t tile = (uniform(H) + 3u) / 4u;
+  tile = tile < 1u ? 1u : (tile > 64u ? 64u : tile);
+  for (uint32_t t0 = wave * tile; t0 < H; t0 += 4u * tile) {
+    const uint32_t t1 = t0 + tile < H ? t0 + tile : H;
+    uint32_t found = 0;  // list length of home atom t0 + lane
+    uint32_t hc = 0;
+    while (hc + 1u < nh && S.hoff[hc + 1u] <= t0) hc++;
+    for (; hc < nh && S.hoff[hc] < t1; hc++) {
+      // values read from LDS are the same in every lane: keep them scalar
+      const uint32_t h0 = uniform(S.hoff[hc]), h1 = uniform(S.hoff[hc + 1u]);
+      const uint32_t k0 = t0 > h0 ? t0 : h0;
+      const uint32_t k1 = t1 < h1 ? t1 : h1;
+      const uint32_t hx = hc % g.gx, hy = hc / g.gx;
+      const uint32_t hs = staged_cell(g, 1u, hy + 1u, hx + 1u);
+      const uint32_t ebase = uniform(S.off[hs]) - h0;                 // staged index of home atom k: ebase + k
+      const uint32_t lbase = uniform(S.start[hs]) - R.atom_off - h0;  // its replica slot: lbase + k
+      for (uint32_t dz = 0; dz < 3u; dz++)
+        for (uint32_t dy = 0; dy < 3u; dy++) {
+          const uint32_t sc = staged_cell(g, dz, hy + dy, hx);
+          const uint32_t q1 = uniform(S.off[sc + 3u]);
+          for (uint32_t c0 = uniform(S.off[sc]); c0 < q1; c0 += 64u) {
+            const uint32_t q = c0 + lane;
+            const bool valid = q < q1;
+            double cx = kFar, cy = kFar, cz = kFar;
+            if (staged) {
+              const uint32_t qs = valid ? q : kListPad;
+              cx = S.x[qs];
+              cy = S.y[qs];
+              cz = S.z[qs];
+            } else if (valid) {
+              staged_pos(R, S, false, q, X, Y, Z, cx, cy, cz);
+            }
+            for (uint32_t k = k0; k < k1; k++) {
+              const uint32_t e = ebase + k;
+              double xi, yi, zi;
+              staged_pos(R, S, staged, e, X, Y, Z, xi, yi, zi);
+              const bool hit = q != e && dist2(cx - xi, cy - yi, cz - zi) < R.list_r2;
+              const unsigned long long mask = __builtin_amdgcn_ballot_w64(hit);
+              if (mask == 0ull) continue;
+              const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)found, (int)(k - t0));
+              const uint32_t pos = f + lane_rank(mask);
+              uint16_t* lst = B.nbr + list_base(R, lbase + k);
+              if (hit && pos < cap) lst[((pos >> 3) << 9) | (pos & 7u)] = (uint16_t)q;
+              found += lane == k - t0 ? (uint32_t)__builtin_popcountll(mask) : 0u;
+            }
+          }
+        }
+    }
+    // Past the capacity the list keeps its first `cap` entries and the
+    // overflow is reported (lennard_jones.wgsl:494-497). The last octet is
+    // filled up with pads.
+    if (t0 + lane < t1) {
+      const Home a = home_atom(g, S, t0 + lane);
+      const uint32_t n = found < cap ? found : cap;
+      uint16_t* lst = B.nbr + list_base(R, a.slot - R.atom_off);
+      for (uint32_t p = n; (p & 7u) != 0u; p++) lst[list_entry(p)] = (uint16_t)kListPad;
+      B.nbr_count[a.slot] = n;
+      if (found > cap) atomicMax(&B.ctl[g.rep].overflow, found);
+    }
   }
 }
 
@@ -603,33 +703,51 @@ struct PairAcc {
   uint32_t near, within;
 };
 
+// 1 / x correctly rounded, for x whose reciprocal is a normal number
+// (here r^2, from about 1e-4 to the sentinel's 3e36): the operations of the
+// IEEE division sequence (reciprocal estimate, two Newton-Raphson steps, one
+// residual correction) without its range scaling and special-value fix-up,
+// which change nothing for such x.
+__device__ __forceinline__ double rcp_rn(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-x, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-x, y, 1.0);
+  return __builtin_fma(e, y, y);
+}
+
 // R5 (FP64) and R5f (FP32 pair arithmetic on FP64 displacements), without
 // branches: the terms of a pair outside the cutoff are computed and then
 // replaced by zero, which leaves every sum bit for bit as if the pair had
 // been skipped (no sum here is ever -0, so adding +0 changes nothing).
+// Forces: G += d fscal with fscal = g / r^2, g = r.F / 24, and a = -24 G
+// once per atom. ENERGIES: the measurement's pe, w, near and within (R9)
+// instead of the force.
 template <int PREC, bool SF, bool ENERGIES>
 __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, double dy, double dz,
                                                 double r2, PairAcc& a) {
   const bool hit = r2 < R.rc2;
   if (PREC == 0) {
-    const double inv2 = 1.0 / r2;
+    const double inv2 = rcp_rn(r2);
     const double s6 = inv2 * inv2 * inv2;
-    double rf = s6 * __builtin_fma(48.0, s6, -24.0);
-    double u = 0.0;
-    if (ENERGIES) {
+    if (!ENERGIES) {
+      double g = s6 * __builtin_fma(s6, 2.0, -1.0);
+      if (SF) g = __builtin_fma(-__builtin_sqrt(r2), R.force_shift24, g);
+      const double fscal = hit ? g * inv2 : 0.0;
+      a.ax = __builtin_fma(dx, fscal, a.ax);
+      a.ay = __builtin_fma(dy, fscal, a.ay);
+      a.az = __builtin_fma(dz, fscal, a.az);
+    } else {
+      double rf = s6 * __builtin_fma(48.0, s6, -24.0);
       const double t = 4.0 * s6;
-      u = __builtin_fma(t, s6, -t) - R.shift;
-    }
-    if (SF) {
-      const double r = __builtin_sqrt(r2);
-      rf = __builtin_fma(-r, R.force_shift, rf);
-      if (ENERGIES) u = __builtin_fma(r - R.rc, R.force_shift, u);
-    }
-    const double fscal = hit ? rf * inv2 : 0.0;
-    a.ax = __builtin_fma(-dx, fscal, a.ax);
-    a.ay = __builtin_fma(-dy, fscal, a.ay);
-    a.az = __builtin_fma(-dz, fscal, a.az);
-    if (ENERGIES) {
+      double u = __builtin_fma(t, s6, -t) - R.shift;
+      if (SF) {
+        const double r = __builtin_sqrt(r2);
+        rf = __builtin_fma(-r, R.force_shift, rf);
+        u = __builtin_fma(r - R.rc, R.force_shift, u);
+      }
       a.pe += hit ? 0.5 * u : 0.0;
       a.w += hit ? 0.5 * rf : 0.0;
     }
@@ -637,23 +755,23 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
     const float q = (float)r2;
     const float inv2 = __builtin_amdgcn_rcpf(q);
     const float s6 = inv2 * inv2 * inv2;
-    float rf = s6 * __builtin_fmaf(48.0f, s6, -24.0f);
-    float u = 0.0f;
-    if (ENERGIES) {
+    if (!ENERGIES) {
+      float g = s6 * __builtin_fmaf(s6, 2.0f, -1.0f);
+      if (SF) g = __builtin_fmaf(-__builtin_sqrtf(q), (float)R.force_shift24, g);
+      const float fscal = hit ? g * inv2 : 0.0f;
+      a.ax += (double)((float)dx * fscal);
+      a.ay += (double)((float)dy * fscal);
+      a.az += (double)((float)dz * fscal);
+    } else {
+      float rf = s6 * __builtin_fmaf(48.0f, s6, -24.0f);
       const float t = 4.0f * s6;
-      u = __builtin_fmaf(t, s6, -t) - (float)R.shift;
-    }
-    if (SF) {
-      const float r = __builtin_sqrtf(q);
-      const float fs = (float)R.force_shift;
-      rf = __builtin_fmaf(-r, fs, rf);
-      if (ENERGIES) u = __builtin_fmaf(r - (float)R.rc, fs, u);
-    }
-    const float fscal = hit ? rf * inv2 : 0.0f;
-    a.ax -= (double)((float)dx * fscal);
-    a.ay -= (double)((float)dy * fscal);
-    a.az -= (double)((float)dz * fscal);
-    if (ENERGIES) {
+      float u = __builtin_fmaf(t, s6, -t) - (float)R.shift;
+      if (SF) {
+        const float r = __builtin_sqrtf(q);
+        const float fs = (float)R.force_shift;
+        rf = __builtin_fmaf(-r, fs, rf);
+        u = __builtin_fmaf(r - (float)R.rc, fs, u);
+      }
       a.pe += hit ? 0.5 * (double)u : 0.0;
       a.w += hit ? 0.5 * (double)rf : 0.0;
     }
@@ -750,14 +868,15 @@ __global__ void __launch_bounds__(kBlock) k_force(Buffers B, int kick) {
     const uint32_t i = at.slot;
     double xi, yi, zi;
     staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
-    const PairAcc a = interactions<PREC, SF, false>(B, R, S, staged, i, xi, yi, zi);
-    B.ax[i] = a.ax;
-    B.ay[i] = a.ay;
-    B.az[i] = a.az;
+    const PairAcc G = interactions<PREC, SF, false>(B, R, S, staged, i, xi, yi, zi);
+    const double ax = -24.0 * G.ax, ay = -24.0 * G.ay, az = -24.0 * G.az;
+    B.ax[i] = ax;
+    B.ay[i] = ay;
+    B.az[i] = az;
     if (kick) {
-      B.vx[i] = __builtin_fma(a.ax, R.half_dt, B.vx[i]);
-      B.vy[i] = __builtin_fma(a.ay, R.half_dt, B.vy[i]);
-      B.vz[i] = __builtin_fma(a.az, R.half_dt, B.vz[i]);
+      B.vx[i] = __builtin_fma(ax, R.half_dt, B.vx[i]);
+      B.vy[i] = __builtin_fma(ay, R.half_dt, B.vy[i]);
+      B.vz[i] = __builtin_fma(az, R.half_dt, B.vz[i]);
     }
   }
 }
@@ -955,6 +1074,12 @@ __global__ void __launch_bounds__(kBlock) k_fixed_sum(const double* values, size
   block_add_slots(v, out);
 }
 
+// rcp_rn over an array (a check of its rounding against IEEE division).
+__global__ void __launch_bounds__(kBlock) k_rcp(const double* x, double* out, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = rcp_rn(x[t]);
+}
+
 __global__ void k_zero_i64(long long* p, uint32_t n) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0;
@@ -989,6 +1114,14 @@ hipError_t launch_drift(const Buffers& B, bool langevin, hipStream_t s) {
   else
     hipLaunchKernelGGL(k_drift<false>, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_decide, decide_grid(B), dim3(kBlock), 0, s, B, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_nh_half(const Buffers& B, bool scale, hipStream_t s) {
+  if (!B.n_atom_blocks) return hipSuccess;
+  hipLaunchKernelGGL(k_ke_sum, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_nh_update, decide_grid(B), dim3(kBlock), 0, s, B);
+  if (scale) hipLaunchKernelGGL(k_nh_scale, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
   return hipGetLastError();
 }
 
@@ -1045,6 +1178,11 @@ hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s) {
     hipLaunchKernelGGL((k_profile<true>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
   else
     hipLaunchKernelGGL((k_profile<false>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_rcp(const double* x, double* out, size_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_rcp, grid1((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, x, out, n);
   return hipGetLastError();
 }
 

@@ -10,10 +10,14 @@ namespace vlj {
 hipError_t launch_decide(const Buffers& B, hipStream_t s);
 hipError_t launch_drift(const Buffers& B, bool langevin, hipStream_t s);
 hipError_t launch_rebuild(const Buffers& B, hipStream_t s);
+// Nose-Hoover half step: kinetic energy, xi / eta / scale; with `scale` the
+// velocities are scaled too (the second half step of a step).
+hipError_t launch_nh_half(const Buffers& B, bool scale, hipStream_t s);
 hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s);
 hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStream_t s);
 hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s);
 hipError_t launch_zero(long long* p, uint32_t n, hipStream_t s);
+hipError_t launch_rcp(const double* x, double* out, size_t n, hipStream_t s);
 hipError_t launch_fixed_sum(const double* values, size_t n, double scale, long long* out,
                             hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t words, hipStream_t s);

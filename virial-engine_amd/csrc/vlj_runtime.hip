@@ -24,6 +24,7 @@ namespace {
 constexpr int kRing = 3;         // read-back slots (lennard_jones.rs:320)
 constexpr uint32_t kGraphSteps = 16;
 constexpr double kTwo32 = 4294967296.0;
+constexpr double kPi = 3.14159265358979323846;  // the f64 nearest pi (std::f64::consts::PI)
 
 thread_local std::string g_error = "no error";
 
@@ -58,6 +59,7 @@ struct vlj_sim {
   int precision = 0;
   bool shifted = false;
   bool any_langevin = false;
+  bool any_nose_hoover = false;
   Buffers B{};
   std::vector<void*> allocations;
   Ring ring[kRing];
@@ -114,10 +116,6 @@ int derive(const vlj_particle* p, size_t n, const vlj_settings& st, Replica& R, 
   if (st.thermostat.kind > VLJ_THERMOSTAT_NOSE_HOOVER) {
     err = "unknown thermostat";
     return VLJ_E_INVALID;
-  }
-  if (st.thermostat.kind == VLJ_THERMOSTAT_NOSE_HOOVER) {
-    err = "the Nose-Hoover thermostat is not built yet";
-    return VLJ_E_UNSUPPORTED;
   }
   if (st.precision > VLJ_PRECISION_FP32) {
     err = "unknown precision";
@@ -190,7 +188,8 @@ int derive(const vlj_particle* p, size_t n, const vlj_settings& st, Replica& R, 
     speed += std::sqrt(v2);
   }
   const bool langevin = st.thermostat.kind == VLJ_THERMOSTAT_LANGEVIN;
-  const double thermal = langevin ? 1.5 * st.thermostat.temperature * (double)n : 0.0;
+  const bool nose_hoover = st.thermostat.kind == VLJ_THERMOSTAT_NOSE_HOOVER;
+  const double thermal = langevin || nose_hoover ? 1.5 * st.thermostat.temperature * (double)n : 0.0;
   H.energy_scale = scale_for(std::max(std::max(kinetic, thermal), (double)n));
   H.momentum_scale = scale_for(std::max(speed, (double)n));
   R.energy_fixed = H.energy_scale * kTwo32;
@@ -202,11 +201,24 @@ int derive(const vlj_particle* p, size_t n, const vlj_settings& st, Replica& R, 
   const double inv6 = 1.0 / (R.rc2 * R.rc2 * R.rc2);
   R.shift = 4.0 * (inv6 * inv6 - inv6);
   R.force_shift = st.shifted_force ? 24.0 * (2.0 * inv6 * inv6 - inv6) / st.cutoff : 0.0;
+  R.force_shift24 = R.force_shift / 24.0;
   R.dt = st.dt;
   R.half_dt = 0.5 * st.dt;
-  R.thermostat = langevin ? 1u : 0u;
+  R.thermostat = langevin ? 1u : (nose_hoover ? 2u : 0u);
   R.c1 = 1.0;
   R.c2 = 0.0;
+  R.nh_q = 1.0;
+  if (nose_hoover) {
+    // lennard_jones.rs:516-528: zero total momentum, g = 3N - 3
+    if (!(st.thermostat.temperature >= 0.0) || !(st.thermostat.period > 0.0)) {
+      err = "Nose-Hoover needs a non-negative temperature and a positive period";
+      return VLJ_E_INVALID;
+    }
+    const double dof = 3.0 * (double)n - 3.0;
+    R.nh_period = st.thermostat.period;
+    R.nh_gkt = dof * st.thermostat.temperature;
+    R.nh_q = dof * st.thermostat.temperature * R.nh_period * R.nh_period / (4.0 * (kPi * kPi));
+  }
   if (langevin) {
     if (!(st.thermostat.temperature >= 0.0) || !(st.thermostat.friction >= 0.0)) {
       err = "thermostat temperature and friction must be non-negative";
@@ -269,10 +281,15 @@ int check_device(vlj_sim* s) {
 }
 
 // Enqueue one step: drift, rebuild chain, forces + kick.
+// One step (lennard_jones.rs:1092-1138): with Nose-Hoover replicas a
+// thermostat half step before the drift (its scaling is applied there) and
+// one after the force kick.
 int enqueue_step(vlj_sim* s) {
+  if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, false, s->stream));
   HIP_TRY(s, launch_drift(s->B, s->any_langevin, s->stream));
   HIP_TRY(s, launch_rebuild(s->B, s->stream));
   HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, true, s->stream));
+  if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, true, s->stream));
   return VLJ_OK;
 }
 
@@ -414,6 +431,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     scan_off += R.scan_blocks;
     list_off += (uint64_t)list_octets(R.capacity) * 8u * ((R.count + kListStride - 1) / kListStride) * kListStride;
     if (R.thermostat == 1) s->any_langevin = true;
+    if (R.thermostat == 2) s->any_nose_hoover = true;
   }
   s->n_total = atom_off;
   s->cells_total = cell_off;
@@ -451,6 +469,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   A(&ob, s->sort_blocks.size());
   A(&gb, s->bricks.size());
   A(&B.bmax, s->atom_blocks.size());
+  A(&B.kepart, s->atom_blocks.size());
   double** d[] = {&B.x, &B.y, &B.z, &B.vx, &B.vy, &B.vz, &B.ax, &B.ay, &B.az,
                   &B.x2, &B.y2, &B.z2, &B.vx2, &B.vy2, &B.vz2, &B.ax2, &B.ay2, &B.az2,
                   &B.xb, &B.yb, &B.zb};
@@ -578,7 +597,8 @@ void decode(const vlj_sim* s, uint32_t r, const long long* slots, const Control&
   m->volume = R.box[0] * R.box[1] * R.box[2];
   m->list_rebuilds = c.rebuilds;
   m->list_overflow = c.overflow;
-  m->thermostat_energy = 0.0;
+  // lennard_jones.rs:1235-1241
+  m->thermostat_energy = R.thermostat == 2 ? 0.5 * R.nh_q * c.xi * c.xi + R.nh_gkt * c.eta : 0.0;
   m->condensed = (uint32_t)slots[6];
   m->pairs_within_cutoff = (uint64_t)slots[7] / 2u;
   for (int k = 0; k < kSlots; k++) m->raw[k] = slots[k];
@@ -770,8 +790,15 @@ int vlj_set_temperature(vlj_sim* sim, double temperature) {
   if (!(temperature >= 0.0)) return fail(sim, VLJ_E_INVALID, "temperature must be non-negative");
   t.temperature = temperature;
   Replica& R = sim->rep[0];
-  const double c1 = std::exp(-t.friction * R.dt);
-  R.c2 = std::sqrt((1.0 - c1 * c1) * temperature);
+  if (t.kind == VLJ_THERMOSTAT_LANGEVIN) {
+    const double c1 = std::exp(-t.friction * R.dt);
+    R.c2 = std::sqrt((1.0 - c1 * c1) * temperature);
+  } else {
+    // Nose-Hoover keeps its period, so Q scales with the temperature (rs:912-920)
+    const double dof = 3.0 * (double)R.count - 3.0;
+    R.nh_gkt = dof * temperature;
+    R.nh_q = R.nh_gkt * R.nh_period * R.nh_period / (4.0 * (kPi * kPi));
+  }
   HIP_TRY(sim, hipMemcpyAsync((void*)&sim->B.rep[0], &R, sizeof(Replica), hipMemcpyHostToDevice, sim->stream));
   HIP_TRY(sim, hipStreamSynchronize(sim->stream));
   return VLJ_OK;
@@ -1054,6 +1081,25 @@ int vlj_copy_bandwidth(size_t bytes, uint32_t repeats, double* ms) {
   return VLJ_OK;
 }
 
+int vlj_debug_rcp(const double* x, double* out, size_t n) {
+  if (n && (!x || !out)) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_debug_rcp");
+  vlj_sim tmp;
+  int rc = check_device(&tmp);
+  if (rc != VLJ_OK) return rc;
+  double *dx = nullptr, *dout = nullptr;
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(double);
+  hipError_t e = hipMalloc(&dx, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dout, bytes);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_rcp(dx, dout, n, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (dx) (void)hipFree(dx);
+  if (dout) (void)hipFree(dout);
+  if (e != hipSuccess) return fail(nullptr, VLJ_E_HIP, std::string("vlj_debug_rcp: ") + hipGetErrorString(e));
+  return VLJ_OK;
+}
+
 int vlj_timed_run(vlj_sim* sim, uint64_t steps, double* force_ms, double* total_ms) {
   if (!sim || !force_ms || !total_ms) return fail(sim, VLJ_E_INVALID, "bad arguments to vlj_timed_run");
   int rc = prime_if_needed(sim);
@@ -1175,6 +1221,106 @@ int vlj_fcc(uint32_t n, double density, double temperature, uint64_t seed, vlj_p
         }
   zero_momentum(i, out);
   *side = L;
+  return VLJ_OK;
+}
+
+// FCC sites for any count: the smallest 4 n^3 lattice that holds `count`
+// atoms, filled in (cz, cy, cx, basis) order, in a cube of side
+// cbrt(count / density) rounded to f32; sites offset by a/4 as in vlj_fcc.
+int vlj_fcc_count(size_t count, double density, double temperature, uint64_t seed, vlj_particle* out,
+                  double* side) {
+  if ((count && !out) || !side || !(density > 0.0)) return fail(nullptr, VLJ_E_INVALID, "bad arguments to fcc_count");
+  uint32_t n = 1;
+  while (4ull * n * n * n < count) n++;
+  const double L = (double)(float)std::cbrt((double)count / density);
+  const double a = L / (double)n;
+  static const double basis[4][3] = {{0, 0, 0}, {0.5, 0.5, 0}, {0.5, 0, 0.5}, {0, 0.5, 0.5}};
+  uint64_t s = seed | 1u;
+  const double st = std::sqrt(temperature);
+  for (size_t i = 0; i < count; i++) {
+    const size_t cell = i / 4, b = i % 4;
+    const double c[3] = {(double)(cell % n), (double)((cell / n) % n), (double)(cell / ((size_t)n * n))};
+    for (int k = 0; k < 3; k++) out[i].pos[k] = (c[k] + basis[b][k] + 0.25) * a;
+    for (int k = 0; k < 3; k++) out[i].vel[k] = lattice_gaussian(&s) * st;
+  }
+  if (count) zero_momentum(count, out);
+  *side = L;
+  return VLJ_OK;
+}
+
+// Random non-overlapping insertion: uniform trial positions from the
+// xorshift stream (seed | 1), rejected when closer than min_distance
+// (nearest image) to an earlier particle, at most 1000 trials per particle;
+// then Gaussian velocities from the same stream, mean removed. A cell grid
+// of cells at least min_distance wide finds the earlier particles that can
+// be that close, so the result is the same as checking every earlier one.
+int vlj_random_insertion(size_t count, const double box[3], double min_distance, double temperature,
+                         uint64_t seed, vlj_particle* out) {
+  if ((count && !out) || !box || !(min_distance > 0.0))
+    return fail(nullptr, VLJ_E_INVALID, "bad arguments to random_insertion");
+  for (int k = 0; k < 3; k++)
+    if (!(box[k] > 0.0)) return fail(nullptr, VLJ_E_INVALID, "box sides must be positive");
+  int nc[3];
+  for (int k = 0; k < 3; k++) {
+    // one cell fewer than fit: cells then exceed min_distance by a margin
+    // far above any rounding of the bucket index
+    const double c = std::floor(box[k] / min_distance) - 1.0;
+    nc[k] = c < 1.0 ? 1 : (c > 1024.0 ? 1024 : (int)c);
+  }
+  std::vector<std::vector<uint32_t>> grid((size_t)nc[0] * nc[1] * nc[2]);
+  auto cell_axis = [&](double x, int k) {
+    int c = (int)std::floor(x / box[k] * nc[k]);
+    return c < 0 ? 0 : (c >= nc[k] ? nc[k] - 1 : c);
+  };
+  uint64_t s = seed | 1u;
+  const double md2 = min_distance * min_distance;
+  std::vector<size_t> near;
+  for (size_t i = 0; i < count; i++) {
+    bool placed = false;
+    for (int trial = 0; trial < 1000 && !placed; trial++) {
+      double p[3];
+      for (int k = 0; k < 3; k++) p[k] = lattice_random(&s) * box[k];
+      int c[3];
+      for (int k = 0; k < 3; k++) c[k] = cell_axis(p[k], k);
+      near.clear();
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            const int x = (c[0] + dx + nc[0]) % nc[0], y = (c[1] + dy + nc[1]) % nc[1],
+                      z = (c[2] + dz + nc[2]) % nc[2];
+            near.push_back(((size_t)z * nc[1] + y) * nc[0] + x);
+          }
+      std::sort(near.begin(), near.end());
+      near.erase(std::unique(near.begin(), near.end()), near.end());
+      placed = true;
+      for (size_t cell : near) {
+        for (uint32_t j : grid[cell]) {
+          double r2 = 0;
+          for (int k = 0; k < 3; k++) {
+            double d = out[j].pos[k] - p[k];
+            d -= box[k] * std::round(d / box[k]);
+            r2 += d * d;
+          }
+          if (r2 < md2) {
+            placed = false;
+            break;
+          }
+        }
+        if (!placed) break;
+      }
+      if (placed) {
+        for (int k = 0; k < 3; k++) out[i].pos[k] = p[k];
+        grid[((size_t)c[2] * nc[1] + c[1]) * nc[0] + c[0]].push_back((uint32_t)i);
+      }
+    }
+    if (!placed)
+      return fail(nullptr, VLJ_E_INVALID,
+                  "random insertion found no free place for particle " + std::to_string(i) + " in 1000 trials");
+  }
+  const double st = std::sqrt(temperature);
+  for (size_t i = 0; i < count; i++)
+    for (int k = 0; k < 3; k++) out[i].vel[k] = lattice_gaussian(&s) * st;
+  if (count) zero_momentum(count, out);
   return VLJ_OK;
 }
 

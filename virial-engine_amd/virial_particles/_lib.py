@@ -110,6 +110,8 @@ SIGNATURES = {
     "vlj_lattice": (C.c_int, [_sz, _d, _d, _u64, _pp]),
     "vlj_lattice_box": (C.c_int, [_sz, _dp, _d, _u64, _pp]),
     "vlj_fcc": (C.c_int, [_u32, _d, _d, _u64, _pp, _dp]),
+    "vlj_fcc_count": (C.c_int, [_sz, _d, _d, _u64, _pp, _dp]),
+    "vlj_random_insertion": (C.c_int, [_sz, _dp, _d, _d, _u64, _pp]),
     "vlj_fixed_sum": (C.c_int, [_dp, _sz, _d, _i64p]),
     "vlj_debug_state": (C.c_int, [_P, _dp, _dp, _dp, _up]),
     "vlj_debug_cells": (C.c_int, [_P, _up, _up, _up]),
@@ -118,6 +120,7 @@ SIGNATURES = {
     "vlj_debug_params": (C.c_int, [_P, _dp]),
     "vlj_debug_swizzle": (_u32, [_u32, _u32]),
     "vlj_copy_bandwidth": (C.c_int, [_sz, _u32, _dp]),
+    "vlj_debug_rcp": (C.c_int, [_dp, _dp, _sz]),
     "vlj_timed_run": (C.c_int, [_P, _u64, _dp, _dp]),
 }
 

@@ -564,6 +564,31 @@ def fcc_array(cells_per_side: int, density: float, temperature: float, seed: int
     return out, side.value
 
 
+def fcc_count_array(count: int, density: float, temperature: float, seed: int):
+    """FCC sites for any count (the first `count` of the smallest 4 n^3
+    lattice); returns (particles, side)."""
+    out = np.zeros(count, dtype=PARTICLE_DTYPE)
+    side = C.c_double()
+    _check(L.lib().vlj_fcc_count(count, density, temperature, seed, _pptr(out), C.byref(side)))
+    return out, side.value
+
+
+def random_insertion_array(count: int, box, min_distance: float, temperature: float, seed: int) -> np.ndarray:
+    """Random non-overlapping insertion (uniform trials, min_distance apart)."""
+    out = np.zeros(count, dtype=PARTICLE_DTYPE)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    _check(L.lib().vlj_random_insertion(count, _dptr(b), min_distance, temperature, seed, _pptr(out)))
+    return out
+
+
+def debug_rcp(x) -> np.ndarray:
+    """The force loop's reciprocal of each x, computed on the device."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    _check(L.lib().vlj_debug_rcp(_dptr(x), _dptr(out), x.shape[0]))
+    return out
+
+
 def copy_bandwidth(nbytes: int, repeats: int = 10) -> float:
     """Average ms of a plain device copy of nbytes (read once, written once)."""
     ms = C.c_double()
