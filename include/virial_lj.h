@@ -158,6 +158,27 @@ int vlj_read_profile_raw(vlj_sim *sim, int64_t *raw, size_t capacity, uint64_t *
 int vlj_batch_read_profile_raw(vlj_batch *batch, size_t replica, int64_t *raw, size_t capacity,
                                uint64_t *samples);
 
+/* Heat exchange, eHEX/a (lennard_jones.rs:927-943, lennard_jones.wgsl:732-870):
+ * after every step the slab hot[0] <= z < hot[1] gains `energy` and the
+ * slab cold[0] <= z < cold[1] loses it, by scaling velocities about each
+ * slab's centre-of-mass velocity with the O(dt^3) position correction of
+ * Wirnsberger, Frenkel and Dellago (2015). on = 0 stops it. VLJ_E_STATE
+ * "heat exchange runs at constant energy, without a thermostat" with one. */
+int vlj_set_heat_exchange(vlj_sim *sim, int32_t on, const double hot[2], const double cold[2],
+                          double energy);
+/* heat_flux (rs:945-951): energy / (2 Lx Ly dt); *has = 0 when off. */
+int vlj_heat_flux(const vlj_sim *sim, double *flux, int32_t *has);
+/* Heat-current series for Green-Kubo (rs:953-1012, lennard_jones.wgsl:872-947):
+ * J = sum_i e_i v_i + 1/2 sum_ij (F_ij . v_i) r_ij. record makes room for
+ * `capacity` samples (dropping any); sample adds one (VLJ_E_STATE "record_heat_current
+ * first", "heat current series full"); read writes 3 values (x, y, z) per
+ * sample, capacity counted in samples; read_raw the fixed-point sums
+ * (value * 2^32). */
+int vlj_record_heat_current(vlj_sim *sim, uint32_t capacity);
+int vlj_sample_heat_current(vlj_sim *sim);
+int vlj_read_heat_current(vlj_sim *sim, double *out, size_t capacity, uint32_t *samples);
+int vlj_read_heat_current_raw(vlj_sim *sim, int64_t *out, size_t capacity, uint32_t *samples);
+
 /* Last error message of the handle (or of the last failed vlj_create when
  * sim is NULL). Never NULL. */
 const char *vlj_last_error(const vlj_sim *sim);
@@ -218,13 +239,20 @@ int vlj_debug_cells(vlj_sim *sim, uint32_t *cell_of, uint32_t *cell_start, uint3
 int vlj_debug_lists(vlj_sim *sim, uint32_t *counts, uint32_t *entries);
 /* Derived integers of replica 0: cells[3], total_cells, capacity, rebuilds. */
 int vlj_debug_grid(const vlj_sim *sim, uint32_t out[6]);
+/* Totals over the handle's replicas: list rebuilds, brick list builds that
+ * staged from global memory (the neighbourhood outgrew LDS), the largest
+ * list overflow, and the number of bricks. No counterpart in the reference. */
+int vlj_debug_counters(const vlj_sim *sim, uint64_t out[4]);
 /* Derived reals of replica 0: energy_scale, momentum_scale, shift,
  * force_shift, c1, c2. */
 int vlj_debug_params(const vlj_sim *sim, double out[6]);
 
-/* Table entry that workgroup `block` of a `blocks`-workgroup launch works
- * on: workgroups go round-robin to the 8 XCDs, and each XCD gets one
- * contiguous share of the table. A bijection on [0, blocks). */
+/* Workgroup-to-table mapping (no counterpart in the reference). With
+ * blocks != 0: the table entry workgroup `block` works on -- workgroups go
+ * round-robin to the 8 XCDs, and the table is dealt to them in chunks of 16
+ * consecutive entries, chunk c to XCD c % 8. With blocks == 0: the grid
+ * launched for a table of `block` entries (rounded up to whole rounds of
+ * 128 workgroups; those past the end return at once). */
 uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks);
 
 /* The force loop's reciprocal 1 / x for x in [1e-4, 1e37] (n values), for

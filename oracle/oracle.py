@@ -121,6 +121,12 @@ def lib() -> C.CDLL:
         L.vo_rebuilds.argtypes = [P]
         L.vo_rebuilds.restype = u32
         L.vo_set_profile_bins.argtypes = [P, u32]
+        L.vo_set_heat_exchange.argtypes = [P, i32, dp, dp, d]
+        L.vo_set_heat_exchange.restype = i32
+        L.vo_record_heat_current.argtypes = [P, u32]
+        L.vo_sample_heat_current.argtypes = [P]
+        L.vo_sample_heat_current.restype = i32
+        L.vo_read_heat_current.argtypes = [P, C.POINTER(i64), C.POINTER(u32)]
         L.vo_clear_profile.argtypes = [P]
         L.vo_sample_profile.argtypes = [P]
         L.vo_read_profile.argtypes = [P, C.POINTER(i64), C.POINTER(u64)]
@@ -334,6 +340,29 @@ class Sim:
 
     def rebuilds(self) -> int:
         return lib().vo_rebuilds(self._h)
+
+    # Heat exchange and heat current (lennard_jones.wgsl:732-947)
+    def set_heat_exchange(self, hot, cold, energy):
+        on = hot is not None
+        h = np.ascontiguousarray(hot if on else [0.0, 0.0], dtype=np.float64)
+        c = np.ascontiguousarray(cold if on else [0.0, 0.0], dtype=np.float64)
+        if lib().vo_set_heat_exchange(self._h, int(on), _dp(h), _dp(c), float(energy)) != 0:
+            raise ValueError("heat exchange runs at constant energy, without a thermostat")
+
+    def record_heat_current(self, capacity: int):
+        self.hc_capacity = int(capacity)
+        lib().vo_record_heat_current(self._h, self.hc_capacity)
+
+    def sample_heat_current(self):
+        if lib().vo_sample_heat_current(self._h) != 0:
+            raise ValueError("heat current series full or not recorded")
+
+    def read_heat_current(self):
+        """Raw (samples, 3) int64 sums, value * 2^32."""
+        raw = np.zeros((max(self.hc_capacity, 1), 3), dtype=np.int64)
+        n = C.c_uint32(0)
+        lib().vo_read_heat_current(self._h, raw.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(n))
+        return raw[: n.value]
 
     # Profiles along z (lennard_jones.wgsl:628-730): raw fixed-point sums,
     # bins * 8 + 3 of them, and the sample count.

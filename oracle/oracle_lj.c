@@ -80,6 +80,10 @@ struct vo_sim {
     double max_moved2;
     uint64_t steps;
     double xi, eta, nh_sm1; /* Nose-Hoover friction, its integral, last scale - 1 */
+    int hex_on;              /* heat exchange (R15) */
+    double hex_slabs[4], hex_energy;
+    uint32_t hc_cap, hc_samples; /* heat-current series (R14) */
+    int64_t *hc;
 };
 
 /* ------------------------------------------------------------------ */
@@ -669,6 +673,7 @@ void vo_destroy(vo_sim *s)
     free(s->near);
     free(s->within);
     free(s->prof);
+    free(s->hc);
     free(s);
 }
 
@@ -998,6 +1003,172 @@ static void nh_scale(vo_sim *s)
             s->v[k][i] = fma(s->v[k][i], sm1, s->v[k][i]);
 }
 
+/* R15 heat exchange, eHEX/a (Wirnsberger, Frenkel and Dellago 2015, eq. 20;
+ * lennard_jones.wgsl:732-870): after the step the hot slab gains `energy`
+ * and the cold slab loses it, by scaling velocities about each slab's
+ * centre-of-mass velocity, with the O(dt^3) position correction. Slab sums
+ * go through 64-bit fixed point (count exact; momentum and force at the
+ * momentum scale; v^2 and f.v at the energy scale). Membership uses the
+ * wrapped z. */
+static void pair_terms(const vo_params *p, double r2, double *rf_out, double *u_out);
+
+static int hex_slab(const vo_sim *s, double z)
+{
+    const double *b = s->hex_slabs;
+    if (z >= b[0] && z < b[1])
+        return 0;
+    if (z >= b[2] && z < b[3])
+        return 1;
+    return 2;
+}
+
+static void heat_exchange(vo_sim *s)
+{
+    const vo_params *p = &s->p;
+    int32_t n = (int32_t)p->count;
+    int64_t cnt0 = 0, cnt1 = 0, q0[8] = {0}, q1[8] = {0};
+#pragma omp parallel for schedule(static) reduction(+ : cnt0, cnt1) reduction(+ : q0[:8], q1[:8])
+    for (int32_t i = 0; i < n; i++) {
+        int slab = hex_slab(s, wrap_step(s->x[2][i], p->box[2]));
+        if (slab == 2)
+            continue;
+        double v[3] = {s->v[0][i], s->v[1][i], s->v[2][i]};
+        double f[3] = {s->a[0][i], s->a[1][i], s->a[2][i]};
+        int64_t *q = slab == 0 ? q0 : q1;
+        if (slab == 0)
+            cnt0++;
+        else
+            cnt1++;
+        for (int k = 0; k < 3; k++) {
+            q[k] += vo_fixed_f64(v[k], p->momentum_fixed);
+            q[5 + k] += vo_fixed_f64(f[k], p->momentum_fixed);
+        }
+        q[3] += vo_fixed_f64(fma(v[2], v[2], fma(v[1], v[1], v[0] * v[0])), p->energy_fixed);
+        q[4] += vo_fixed_f64(fma(f[2], v[2], fma(f[1], v[1], f[0] * v[0])), p->energy_fixed);
+    }
+    double com[2][3], mf[2][3], sm1[2], cvel[2], cforce[2];
+    const double dt3 = p->dt * p->dt * p->dt;
+    for (int slab = 0; slab < 2; slab++) {
+        const int64_t *q = slab == 0 ? q0 : q1;
+        double m = (double)(slab == 0 ? cnt0 : cnt1);
+        sm1[slab] = cvel[slab] = cforce[slab] = 0.0;
+        for (int k = 0; k < 3; k++)
+            com[slab][k] = mf[slab][k] = 0.0;
+        if (!(m > 0.0))
+            continue;
+        double pm[3], fm[3];
+        for (int k = 0; k < 3; k++) {
+            pm[k] = (double)q[k] / p->momentum_fixed;
+            fm[k] = (double)q[5 + k] / p->momentum_fixed;
+            com[slab][k] = pm[k] / m;
+            mf[slab][k] = fm[k] / m;
+        }
+        double v2 = (double)q[3] / p->energy_fixed;
+        double fv = (double)q[4] / p->energy_fixed;
+        double pp = fma(pm[2], pm[2], fma(pm[1], pm[1], pm[0] * pm[0]));
+        double kinetic = 0.5 * v2 - 0.5 * pp / m;
+        double energy = slab == 0 ? s->hex_energy : -s->hex_energy;
+        double flux = energy / p->dt;
+        double x = energy / kinetic;
+        sm1[slab] = x / (1.0 + sqrt(1.0 + x));
+        double power = fv - fma(fm[2], com[slab][2], fma(fm[1], com[slab][1], fm[0] * com[slab][0]));
+        cvel[slab] = dt3 * flux / (2.0 * kinetic * kinetic) * (flux / 48.0 + power / 6.0);
+        cforce[slab] = dt3 * flux / (12.0 * kinetic);
+    }
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; i++) {
+        int slab = hex_slab(s, wrap_step(s->x[2][i], p->box[2]));
+        if (slab == 2)
+            continue;
+        for (int k = 0; k < 3; k++) {
+            double rel = s->v[k][i] - com[slab][k];
+            double shift = cvel[slab] * rel - cforce[slab] * (s->a[k][i] - mf[slab][k]);
+            s->x[k][i] = s->x[k][i] - shift;
+            s->v[k][i] = fma(rel, sm1[slab], s->v[k][i]);
+        }
+    }
+}
+
+int vo_set_heat_exchange(vo_sim *s, int on, const double hot[2], const double cold[2], double energy)
+{
+    if (s->p.thermostat != 0)
+        return -1;
+    s->hex_on = on;
+    s->hex_slabs[0] = on ? hot[0] : 0.0;
+    s->hex_slabs[1] = on ? hot[1] : 0.0;
+    s->hex_slabs[2] = on ? cold[0] : 0.0;
+    s->hex_slabs[3] = on ? cold[1] : 0.0;
+    s->hex_energy = on ? energy : 0.0;
+    return 0;
+}
+
+/* R14 heat current J = sum_i e_i v_i + 1/2 sum_i sum_j (F_ij . v_i) r_ij
+ * (lennard_jones.wgsl:872-947), e_i = KE_i + half of each pair energy, over
+ * atom i's list in order: q = ((d . v_i) rf) / r2, P = fma(d, q, P);
+ * j_i = fma(e_i, v_i, 0.5 P); each component to fixed point at 2^32 and
+ * summed as integers, one sample of three sums. */
+void vo_record_heat_current(vo_sim *s, uint32_t capacity)
+{
+    free(s->hc);
+    s->hc = capacity ? (int64_t *)xcalloc((size_t)capacity * 3, sizeof(int64_t)) : NULL;
+    s->hc_cap = capacity;
+    s->hc_samples = 0;
+}
+
+int vo_sample_heat_current(vo_sim *s)
+{
+    if (!s->hc || s->hc_samples >= s->hc_cap)
+        return -1;
+    vo_prime(s);
+    const vo_params *p = &s->p;
+    int32_t n = (int32_t)p->count;
+    int64_t jx = 0, jy = 0, jz = 0;
+#pragma omp parallel for schedule(static) reduction(+ : jx, jy, jz)
+    for (int32_t i = 0; i < n; i++) {
+        const double xi = s->x[0][i], yi = s->x[1][i], zi = s->x[2][i];
+        const double v[3] = {s->v[0][i], s->v[1][i], s->v[2][i]};
+        double e = 0.5 * fma(v[2], v[2], fma(v[1], v[1], v[0] * v[0]));
+        double pr[3] = {0.0, 0.0, 0.0};
+        const uint32_t cnt = s->nbr_count[i];
+        const uint32_t *row = s->nbr + (size_t)i * p->capacity;
+        const uint8_t *img = s->nbr_image + (size_t)i * p->capacity;
+        for (uint32_t k = 0; k < cnt; k++) {
+            const uint32_t j = row[k];
+            const int code = img[k];
+            const double d[3] = {(s->x[0][j] + (code % 3 - 1) * p->box[0]) - xi,
+                                 (s->x[1][j] + ((code / 3) % 3 - 1) * p->box[1]) - yi,
+                                 (s->x[2][j] + (code / 9 - 1) * p->box[2]) - zi};
+            const double r2 = fma(d[2], d[2], fma(d[1], d[1], d[0] * d[0]));
+            if (!(r2 < p->rc2))
+                continue;
+            double rf, u;
+            pair_terms(p, r2, &rf, &u);
+            e += 0.5 * u;
+            const double dv = fma(d[2], v[2], fma(d[1], v[1], d[0] * v[0]));
+            const double q = dv * rf / r2;
+            for (int a = 0; a < 3; a++)
+                pr[a] = fma(d[a], q, pr[a]);
+        }
+        jx += llrint(fma(e, v[0], 0.5 * pr[0]) * TWO32);
+        jy += llrint(fma(e, v[1], 0.5 * pr[1]) * TWO32);
+        jz += llrint(fma(e, v[2], 0.5 * pr[2]) * TWO32);
+    }
+    int64_t *out = s->hc + 3 * (size_t)s->hc_samples;
+    out[0] = jx;
+    out[1] = jy;
+    out[2] = jz;
+    s->hc_samples++;
+    return 0;
+}
+
+void vo_read_heat_current(const vo_sim *s, int64_t *raw, uint32_t *samples)
+{
+    if (raw && s->hc)
+        memcpy(raw, s->hc, (size_t)s->hc_samples * 3 * sizeof(int64_t));
+    if (samples)
+        *samples = s->hc_samples;
+}
+
 /* lennard_jones.rs:1092-1138 (NVE, Langevin, Nose-Hoover) */
 void vo_run(vo_sim *s, uint64_t steps)
 {
@@ -1013,6 +1184,8 @@ void vo_run(vo_sim *s, uint64_t steps)
             nh_half(s);
             nh_scale(s);
         }
+        if (s->hex_on)
+            heat_exchange(s);
         s->steps++;
     }
 }

@@ -115,6 +115,16 @@ void vo_clear_profile(vo_sim *s);
 void vo_sample_profile(vo_sim *s);
 void vo_read_profile(const vo_sim *s, int64_t *raw, uint64_t *samples);
 
+/* Heat exchange, eHEX/a (lennard_jones.wgsl:732-870; rs:927-950): -1 with a
+ * thermostat. on = 0 stops it. */
+int vo_set_heat_exchange(vo_sim *s, int on, const double hot[2], const double cold[2], double energy);
+/* Heat-current series (lennard_jones.wgsl:872-947; rs:952-1012): raw holds
+ * 3 int64 (value * 2^32) per sample. sample returns -1 when full or not
+ * recorded. */
+void vo_record_heat_current(vo_sim *s, uint32_t capacity);
+int vo_sample_heat_current(vo_sim *s);
+void vo_read_heat_current(const vo_sim *s, int64_t *raw, uint32_t *samples);
+
 /* Brute-force all pairs, nearest image (tests/thermodynamics.rs:20-53,
  * tests/coexistence.rs:38-76). Any output pointer may be NULL. */
 void vo_all_pairs(const double *pos, size_t n, const double box[3], double cutoff,

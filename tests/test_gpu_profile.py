@@ -1,7 +1,7 @@
 """GPU profiles along z (lennard_jones.wgsl:628-730) against the oracle.
 
 Every (pair, slab, axis) share goes to 64-bit fixed point on its own in both,
-so the raw sums are compared bit for bit: the LDS path (up to 128 slabs),
+so the raw sums are compared bit for bit: the LDS path (up to 256 slabs),
 the global-atomic path (more slabs), shifted forces, a box of two cells per
 axis, the global-memory staging fallback, and batched replicas.
 """
@@ -24,7 +24,7 @@ def pair(oracle, lj, pos, vel, box, bins, cutoff=2.5, dt=0.005, shifted_force=Fa
     return sim, ref
 
 
-@pytest.mark.parametrize("bins", [1, 7, 40, 128, 129, 300])
+@pytest.mark.parametrize("bins", [1, 7, 40, 200, 256, 257, 300])
 def test_profile_sums_bit_exact(oracle, lj, bins):
     print(f"profile parity, bins {bins}", flush=True)
     pos, vel, box = sc_liquid(oracle, n_side=12)

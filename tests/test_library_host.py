@@ -127,17 +127,21 @@ def test_lattice_api_mirror(lj):
     assert ps[0].pos == [1.0, 1.0, 1.0]
 
 
-@pytest.mark.parametrize("n", [1, 7, 8, 9, 54, 4096, 5407, 65536])
-def test_xcd_swizzle_is_a_bijection(n):
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 54, 127, 128, 129, 4096, 5407, 65536])
+def test_table_dealing_covers_every_entry_once(n):
     from virial_particles import _lib
 
     f = _lib.lib().vlj_debug_swizzle
-    m = [f(b, n) for b in range(n)]
-    assert sorted(m) == list(range(n))
-    # each XCD (b % 8) gets one contiguous share
-    for x in range(8):
-        share = sorted(m[b] for b in range(x, n, 8))
-        assert share == list(range(share[0], share[0] + len(share))) if share else True
+    grid = f(n, 0)
+    assert grid % 128 == 0 and n <= grid < n + 128
+    m = [f(b, 1) for b in range(grid)]
+    assert sorted(m) == list(range(grid))
+    # chunks of 16 consecutive entries, chunk c on XCD c % 8 (= b % 8)
+    for b, idx in enumerate(m):
+        assert (idx // 16) % 8 == b % 8
+    # a stretch of 128 consecutive entries touches all eight XCDs
+    if n >= 128:
+        assert {b % 8 for b, idx in enumerate(m) if idx < 128} == set(range(8))
 
 
 def test_measurement_helpers(lj):

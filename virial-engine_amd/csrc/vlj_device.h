@@ -22,9 +22,12 @@ constexpr int kScanCells = 1024;  // cells per scan block (256 threads x 4)
 constexpr int kSortCells = 4;     // cells per sort block (one wave64 per cell)
 constexpr uint32_t kProfileSums = 8;       // sums per profile slab (lennard_jones.wgsl:644)
 constexpr uint32_t kMaxProfileBins = 1u << 20;  // profile slabs per replica, at most
-constexpr int kBrickMax = 6;      // home cells of a brick along x and along y, at most
+constexpr uint32_t kHexSums = 18;   // per slab (hot, cold): count, momentum x3, v^2, f.v, force x3
+constexpr uint32_t kHexState = 18;  // per slab: com x3, mean force x3, s - 1, c_velocity, c_force
+constexpr int kBrickMax = 8;      // home cells of a brick along x and along y, at most
 constexpr int kStageCells = 3 * (kBrickMax + 2) * (kBrickMax + 2);  // staged cells, at most
 constexpr int kHomeCells = kBrickMax * kBrickMax;                    // home cells, at most
+static_assert(kStageCells <= 2 * kBlock, "staging takes two cells per thread");
 constexpr int kStageMax = 2048;   // atoms a brick workgroup stages in LDS
 constexpr int kListStride = 64;   // lists are transposed per block of 64 atoms (one wave)
 constexpr int kSlots = 8;         // fixed-point slots per replica
@@ -52,6 +55,10 @@ struct Replica {
   uint32_t near_count;
   uint32_t index;
   double nh_gkt, nh_q, nh_period;  // Nose-Hoover g k T, mass Q and period (thermostat 2)
+  double hex_slabs[4];  // heat exchange: hot [0, 1), cold [2, 3) in z
+  double hex_energy;    // energy moved per step
+  uint32_t hex_on;      // heat exchange on
+  uint32_t hc_cap;      // heat-current samples the series holds (0: none)
   double bin_width;    // profile slab height, L_z / prof_bins
   uint32_t prof_bins;  // profile slabs along z (0: no profiles)
   uint32_t prof_off;   // first of the replica's prof_bins * 8 + 3 profile sums
@@ -67,7 +74,7 @@ struct Control {
   uint32_t rebuilds;
   uint32_t overflow;
   uint32_t built;
-  uint32_t pad;
+  uint32_t stage_overflow;        // brick list builds that staged from global memory
   double xi, eta, sm1;            // Nose-Hoover friction, its integral, velocity scale - 1
 };
 
@@ -93,6 +100,9 @@ struct Buffers {
   uint32_t n_atom_blocks, n_scan_blocks, n_sort_blocks, n_bricks, n_replicas;
   unsigned long long* bmax;   // per atom block: max moved^2 (FP64 bits)
   long long* kepart;          // per atom block: kinetic energy in fixed point (Nose-Hoover)
+  long long* hexpart;         // per atom block: kHexSums heat-exchange sums
+  double* hexstate;           // per replica: kHexState scales and corrections
+  long long* hc;              // heat-current series of replica 0: 3 sums per sample
   double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;           // state, slot order
   double *x2, *y2, *z2, *vx2, *vy2, *vz2, *ax2, *ay2, *az2;  // reorder target
   double *xb, *yb, *zb;                                      // position at last list build
@@ -109,11 +119,22 @@ struct Buffers {
 };
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
-// XCD b % 8). This maps them so that each XCD gets one contiguous share of
-// the table, whose atoms and neighbours then stay in that XCD's L2.
-__host__ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t n) {
-  const uint32_t q = n / 8u, r = n % 8u, xcd = b % 8u, idx = b / 8u;
-  return xcd < r ? xcd * (q + 1u) + idx : r * (q + 1u) + (xcd - r) * q + idx;
+// XCD b % 8). Each table (atom blocks, sort blocks, bricks) is dealt to the
+// XCDs in chunks of kChunk consecutive entries, chunk c to XCD c % 8:
+// neighbouring entries (adjacent bricks, consecutive atoms) share an XCD and
+// its L2, and any stretch longer than 8 kChunk entries -- one replica's
+// rebuild, the dense quarter of a slab -- is spread over all eight XCDs. The
+// grid is rounded up to whole rounds of 8 kChunk workgroups; workgroups past
+// the table's end return at once.
+constexpr uint32_t kChunk = 16;
+
+__host__ __device__ __forceinline__ uint32_t dealt_grid(uint32_t n) {
+  return (n + 8u * kChunk - 1u) / (8u * kChunk) * (8u * kChunk);
+}
+
+__host__ __device__ __forceinline__ uint32_t dealt_index(uint32_t b) {
+  const uint32_t xcd = b % 8u, j = b / 8u;
+  return ((j / kChunk) * 8u + xcd) * kChunk + j % kChunk;
 }
 
 // A list entry is the neighbour's index in its brick's staging order (the

@@ -113,7 +113,8 @@ __global__ void __launch_bounds__(kBlock) k_decide(Buffers B, int advance) {
 // LANGEVIN: some replica of the launch uses BAOAB (others stay NVE).
 template <bool LANGEVIN>
 __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
-  const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t b = dealt_index(blockIdx.x);
+  if (b >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[b];
   const Replica R = B.rep[blk.rep];
   const uint32_t i = blk.first + threadIdx.x;
@@ -170,7 +171,8 @@ __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(kBlock) k_ke_sum(Buffers B) {
-  const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t b = dealt_index(blockIdx.x);
+  if (b >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[b];
   const Replica R = B.rep[blk.rep];
   if (R.thermostat != 2u) return;
@@ -218,7 +220,9 @@ __global__ void __launch_bounds__(kBlock) k_nh_update(Buffers B) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_nh_scale(Buffers B) {
-  const Range blk = B.atom_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const uint32_t ab = dealt_index(blockIdx.x);
+  if (ab >= B.n_atom_blocks) return;
+  const Range blk = B.atom_blocks[ab];
   const Replica R = B.rep[blk.rep];
   if (R.thermostat != 2u) return;
   const uint32_t i = blk.first + threadIdx.x;
@@ -227,6 +231,136 @@ __global__ void __launch_bounds__(kBlock) k_nh_scale(Buffers B) {
   B.vx[i] = __builtin_fma(B.vx[i], sm1, B.vx[i]);
   B.vy[i] = __builtin_fma(B.vy[i], sm1, B.vy[i]);
   B.vz[i] = __builtin_fma(B.vz[i], sm1, B.vz[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Heat exchange, eHEX/a (Wirnsberger, Frenkel and Dellago 2015, eq. 20;
+// lennard_jones.wgsl:732-870, R15): after the step the hot slab gains
+// hex_energy and the cold slab loses it, by scaling velocities about each
+// slab's centre-of-mass velocity, with the O(dt^3) position correction.
+// Slab sums in 64-bit fixed point per atom block; per replica the scales.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int hex_slab(const Replica& R, double z) {
+  const double zw = wrap_one(z, R.box[2]);
+  if (zw >= R.hex_slabs[0] && zw < R.hex_slabs[1]) return 0;
+  if (zw >= R.hex_slabs[2] && zw < R.hex_slabs[3]) return 1;
+  return 2;
+}
+
+__global__ void __launch_bounds__(kBlock) k_hex_sum(Buffers B) {
+  const uint32_t b = dealt_index(blockIdx.x);
+  if (b >= B.n_atom_blocks) return;
+  const Range blk = B.atom_blocks[b];
+  const Replica R = B.rep[blk.rep];
+  if (!R.hex_on) return;
+  const uint32_t i = blk.first + threadIdx.x;
+  long long q[kHexSums];
+#pragma unroll
+  for (uint32_t k = 0; k < kHexSums; k++) q[k] = 0;
+  if (i < blk.end) {
+    const int slab = hex_slab(R, B.z[i]);
+    if (slab < 2) {
+      const double v[3] = {B.vx[i], B.vy[i], B.vz[i]};
+      const double f[3] = {B.ax[i], B.ay[i], B.az[i]};
+      const long long c = 1;
+      long long t[9];
+      t[0] = c;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        t[1 + k] = to_fixed(v[k], R.momentum_fixed);
+        t[6 + k] = to_fixed(f[k], R.momentum_fixed);
+      }
+      t[4] = to_fixed(dist2(v[0], v[1], v[2]), R.energy_fixed);
+      t[5] = to_fixed(__builtin_fma(f[2], v[2], __builtin_fma(f[1], v[1], f[0] * v[0])), R.energy_fixed);
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        q[k] = slab == 0 ? t[k] : 0;
+        q[9 + k] = slab == 1 ? t[k] : 0;
+      }
+    }
+  }
+  __shared__ long long part[kBlock / 64][kHexSums];
+#pragma unroll
+  for (uint32_t k = 0; k < kHexSums; k++) {
+    const long long w = wave_sum_i64(q[k]);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][k] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < kHexSums) {
+    long long t = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) t += part[w][threadIdx.x];
+    B.hexpart[(size_t)b * kHexSums + threadIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_hex_update(Buffers B) {
+  const uint32_t r = blockIdx.x;
+  const Replica R = B.rep[r];
+  if (!R.hex_on) return;
+  __shared__ long long sums[kHexSums];
+  if (threadIdx.x < kHexSums) {
+    long long t = 0;
+    for (uint32_t b = R.atom_block_off; b < R.atom_block_off + R.atom_blocks; b++)
+      t += B.hexpart[(size_t)b * kHexSums + threadIdx.x];
+    sums[threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double* out = B.hexstate + (size_t)r * kHexState;
+  const double dt3 = R.dt * R.dt * R.dt;
+  for (int slab = 0; slab < 2; slab++) {
+    const long long* q = sums + 9 * slab;
+    double* o = out + 9 * slab;
+    const double m = (double)q[0];
+#pragma unroll
+    for (int k = 0; k < 9; k++) o[k] = 0.0;
+    if (!(m > 0.0)) continue;
+    double pm[3], fm[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      pm[k] = (double)q[1 + k] / R.momentum_fixed;
+      fm[k] = (double)q[6 + k] / R.momentum_fixed;
+      o[k] = pm[k] / m;      // centre-of-mass velocity
+      o[3 + k] = fm[k] / m;  // mean force
+    }
+    const double v2 = (double)q[4] / R.energy_fixed;
+    const double fv = (double)q[5] / R.energy_fixed;
+    const double pp = __builtin_fma(pm[2], pm[2], __builtin_fma(pm[1], pm[1], pm[0] * pm[0]));
+    const double kinetic = 0.5 * v2 - 0.5 * pp / m;
+    const double energy = slab == 0 ? R.hex_energy : -R.hex_energy;
+    const double flux = energy / R.dt;
+    const double x = energy / kinetic;
+    o[6] = x / (1.0 + __builtin_sqrt(1.0 + x));
+    const double power = fv - __builtin_fma(fm[2], o[2], __builtin_fma(fm[1], o[1], fm[0] * o[0]));
+    o[7] = dt3 * flux / (2.0 * kinetic * kinetic) * (flux / 48.0 + power / 6.0);
+    o[8] = dt3 * flux / (12.0 * kinetic);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_hex_apply(Buffers B) {
+  const uint32_t ab = dealt_index(blockIdx.x);
+  if (ab >= B.n_atom_blocks) return;
+  const Range blk = B.atom_blocks[ab];
+  const Replica R = B.rep[blk.rep];
+  if (!R.hex_on) return;
+  const uint32_t i = blk.first + threadIdx.x;
+  if (i >= blk.end) return;
+  const int slab = hex_slab(R, B.z[i]);
+  if (slab == 2) return;
+  const double* o = B.hexstate + (size_t)blk.rep * kHexState + 9 * slab;
+  double* X[3] = {B.x, B.y, B.z};
+  double* V[3] = {B.vx, B.vy, B.vz};
+  const double* A[3] = {B.ax, B.ay, B.az};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double v = V[k][i];
+    const double rel = v - o[k];
+    const double shift = o[7] * rel - o[8] * (A[k][i] - o[3 + k]);
+    X[k][i] = X[k][i] - shift;
+    V[k][i] = __builtin_fma(rel, o[6], v);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -254,7 +388,9 @@ __device__ __forceinline__ uint32_t run_atomic_add(uint32_t* counter, uint32_t k
 }
 
 __global__ void __launch_bounds__(kBlock) k_bin(Buffers B) {
-  const Range blk = B.atom_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const uint32_t ab = dealt_index(blockIdx.x);
+  if (ab >= B.n_atom_blocks) return;
+  const Range blk = B.atom_blocks[ab];
   if (!B.ctl[blk.rep].rebuild) return;
   const Replica R = B.rep[blk.rep];
   const uint32_t i = blk.first + threadIdx.x;
@@ -328,7 +464,9 @@ __device__ __forceinline__ uint32_t cell_first_slot(const Buffers& B, const Repl
 }
 
 __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
-  const Range blk = B.atom_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const uint32_t ab = dealt_index(blockIdx.x);
+  if (ab >= B.n_atom_blocks) return;
+  const Range blk = B.atom_blocks[ab];
   if (!B.ctl[blk.rep].rebuild) return;
   const Replica R = B.rep[blk.rep];
   const uint32_t i = blk.first + threadIdx.x;
@@ -342,7 +480,9 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
 // in the cell, which puts the cell in ascending slot order whatever order
 // the scatter produced (the result of grid.wgsl:129-148's insertion sort).
 __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
-  const Range blk = B.sort_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const uint32_t sb = dealt_index(blockIdx.x);
+  if (sb >= B.n_sort_blocks) return;
+  const Range blk = B.sort_blocks[sb];
   if (!B.ctl[blk.rep].rebuild) return;
   const Replica R = B.rep[blk.rep];
   const uint32_t c = blk.first + (threadIdx.x >> 6);
@@ -374,7 +514,9 @@ __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
 // Slot t takes the atom at sorted[t] (lennard_jones.wgsl:419-452); the copy
 // goes to the second state set and `build` copies it back.
 __global__ void __launch_bounds__(kBlock) k_reorder(Buffers B) {
-  const Range blk = B.atom_blocks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const uint32_t ab = dealt_index(blockIdx.x);
+  if (ab >= B.n_atom_blocks) return;
+  const Range blk = B.atom_blocks[ab];
   if (!B.ctl[blk.rep].reorder) return;
   const uint32_t t = blk.first + threadIdx.x;
   if (t >= blk.end) return;
@@ -445,32 +587,40 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
   const int c0 = (int)g.cell0;
   const int cx0 = c0 % nx, cy0 = (c0 / nx) % ny, cz = c0 / (nx * ny);
   const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
-  const uint32_t ns = 3u * sy * sx;
+  const uint32_t ns = 3u * sy * sx;  // at most kStageCells <= 2 kBlock
   const uint32_t t = threadIdx.x;
-  uint32_t n_t = 0;
-  if (t < ns) {
-    const int xx = (int)(t % sx), yy = (int)((t / sx) % sy), dz = (int)(t / (sx * sy));
-    const int ux = cx0 - 1 + xx, uy = cy0 - 1 + yy, uz = cz - 1 + dz;  // unwrapped
-    const int cx = (ux + nx) % nx, cy = (uy + ny) % ny, ccz = (uz + nz) % nz;
-    const uint32_t cell = R.cell_off + (uint32_t)((ccz * ny + cy) * nx + cx);
-    S.start[t] = B.cell_start[cell];
-    n_t = B.cell_n[cell];
-    // image of the cell seen from the brick (R4)
-    const int ix = ux < 0 ? -1 : (ux >= nx ? 1 : 0);
-    const int iy = uy < 0 ? -1 : (uy >= ny ? 1 : 0);
-    const int iz = uz < 0 ? -1 : (uz >= nz ? 1 : 0);
-    S.image[t] = (uint8_t)((ix + 1) + 3 * (iy + 1) + 9 * (iz + 1));
+  // thread t stages cells 2t and 2t + 1
+  uint32_t n_c[2] = {0u, 0u};
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t c = 2u * t + (uint32_t)h;
+    if (c < ns) {
+      const int xx = (int)(c % sx), yy = (int)((c / sx) % sy), dz = (int)(c / (sx * sy));
+      const int ux = cx0 - 1 + xx, uy = cy0 - 1 + yy, uz = cz - 1 + dz;  // unwrapped
+      const int cx = (ux + nx) % nx, cy = (uy + ny) % ny, ccz = (uz + nz) % nz;
+      const uint32_t cell = R.cell_off + (uint32_t)((ccz * ny + cy) * nx + cx);
+      S.start[c] = B.cell_start[cell];
+      n_c[h] = B.cell_n[cell];
+      // image of the cell seen from the brick (R4)
+      const int ix = ux < 0 ? -1 : (ux >= nx ? 1 : 0);
+      const int iy = uy < 0 ? -1 : (uy >= ny ? 1 : 0);
+      const int iz = uz < 0 ? -1 : (uz >= nz ? 1 : 0);
+      S.image[c] = (uint8_t)((ix + 1) + 3 * (iy + 1) + 9 * (iz + 1));
+    }
   }
   // exclusive scan of the staged counts over the workgroup
   __shared__ uint32_t wtot[kBlock / 64];
+  const uint32_t n_t = n_c[0] + n_c[1];
   const int lane = t & 63, wave = t >> 6;
   const uint32_t incl = wave_incl_scan_u32(n_t, lane);
   if (lane == 63) wtot[wave] = incl;
   __syncthreads();
   uint32_t before = 0;
   for (int w = 0; w < wave; w++) before += wtot[w];
-  if (t < ns) S.off[t] = before + incl - n_t;
-  if (t == ns - 1u) {
+  const uint32_t first = before + incl - n_t;
+  if (2u * t < ns) S.off[2u * t] = first;
+  if (2u * t + 1u < ns) S.off[2u * t + 1u] = first + n_c[0];
+  if (t == kBlock - 1) {
     S.off[ns] = before + incl;
     S.total = before + incl;
     S.ns = ns;
@@ -529,8 +679,12 @@ struct Home {
 };
 
 __device__ __forceinline__ Home home_atom(const Brick& g, const Stage& S, uint32_t h) {
-  uint32_t hc = 0;
-  while (hc + 1u < g.gx * g.gy && S.hoff[hc + 1u] <= h) hc++;
+  // the last home cell whose first home atom is at most h (binary search)
+  uint32_t hc = 0, hi = g.gx * g.gy - 1u;
+  while (hc < hi) {
+    const uint32_t mid = (hc + hi + 1u) >> 1;
+    if (S.hoff[mid] <= h) hc = mid; else hi = mid - 1u;
+  }
   Home a;
   a.hx = hc % g.gx;
   a.hy = hc / g.gx;
@@ -587,7 +741,9 @@ __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
 // On a reorder the workgroup also copies its home atoms' state back from
 // the second set.
 __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
-  const Brick g = B.bricks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const uint32_t gb = dealt_index(blockIdx.x);
+  if (gb >= B.n_bricks) return;
+  const Brick g = B.bricks[gb];
   const Control& ctl = B.ctl[g.rep];
   if (!ctl.rebuild) return;
   const bool moved = ctl.reorder != 0u;
@@ -599,6 +755,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
   const bool staged = stage_brick(B, R, g, X, Y, Z, S);
   const uint32_t total = S.total;
   const uint32_t cap = R.capacity;
+  if (!staged && threadIdx.x == 0) atomicAdd(&B.ctl[g.rep].stage_overflow, 1u);
   const uint32_t nh = g.gx * g.gy;
   const uint32_t H = S.hoff[nh];
   for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
@@ -643,6 +800,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
       const uint32_t h0 = uniform(S.hoff[hc]), h1 = uniform(S.hoff[hc + 1u]);
       const uint32_t k0 = t0 > h0 ? t0 : h0;
       const uint32_t k1 = t1 < h1 ? t1 : h1;
+      if (k0 >= k1) continue;  // an empty home cell
       const uint32_t hx = hc % g.gx, hy = hc / g.gx;
       const uint32_t hs = staged_cell(g, 1u, hy + 1u, hx + 1u);
       const uint32_t ebase = uniform(S.off[hs]) - h0;                 // staged index of home atom k: ebase + k
@@ -858,7 +1016,9 @@ __device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica&
 // workgroup per brick.
 template <int PREC, bool SF>
 __global__ void __launch_bounds__(kBlock) k_force(Buffers B, int kick) {
-  const Brick g = B.bricks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const uint32_t gb = dealt_index(blockIdx.x);
+  if (gb >= B.n_bricks) return;
+  const Brick g = B.bricks[gb];
   const Replica R = B.rep[g.rep];
   __shared__ Stage S;
   const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
@@ -888,16 +1048,19 @@ __global__ void __launch_bounds__(kBlock) k_force(Buffers B, int kick) {
 // atomic (fixed.wgsl:71-105).
 // ---------------------------------------------------------------------------
 
+// Adds slot k of every thread (k < n <= kSlots) into dst[k]: wave sums,
+// then one 64-bit atomic per slot for the workgroup.
+template <int n = kSlots>
 __device__ __forceinline__ void block_add_slots(long long v[kSlots], long long* dst) {
   __shared__ long long part[kBlock / 64][kSlots];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < kSlots; k++) {
+  for (int k = 0; k < n; k++) {
     long long s = wave_sum_i64(v[k]);
     if (lane == 0) part[wave][k] = s;
   }
   __syncthreads();
-  if (threadIdx.x < kSlots) {
+  if (threadIdx.x < (unsigned)n) {
     long long s = 0;
 #pragma unroll
     for (int w = 0; w < kBlock / 64; w++) s += part[w][threadIdx.x];
@@ -907,7 +1070,9 @@ __device__ __forceinline__ void block_add_slots(long long v[kSlots], long long* 
 
 template <int PREC, bool SF>
 __global__ void __launch_bounds__(kBlock) k_measure(Buffers B) {
-  const Brick g = B.bricks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const uint32_t gb = dealt_index(blockIdx.x);
+  if (gb >= B.n_bricks) return;
+  const Brick g = B.bricks[gb];
   const Replica R = B.rep[g.rep];
   __shared__ Stage S;
   const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
@@ -944,12 +1109,12 @@ __global__ void __launch_bounds__(kBlock) k_measure(Buffers B) {
 // Every (pair, slab, axis) share and every per-atom value goes to fixed point
 // on its own as rint(x 2^32) and the integers are added, so the sums do not
 // depend on the order of the additions. Pair terms are FP64 in both precision
-// forms. Up to kProfileLdsBins slabs a workgroup adds into LDS and then once
+// forms. Up to kProfileLdsBins (256) slabs a workgroup adds into LDS and then once
 // per touched sum into the replica's sums; with more slabs it adds straight
 // into them.
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t kProfileLdsBins = 128;
+constexpr uint32_t kProfileLdsBins = 256;
 constexpr double kProfileOne = 4294967296.0;  // fixed point: 2^32 per unit
 
 struct ProfileSink {
@@ -968,7 +1133,9 @@ struct ProfileSink {
 
 template <bool SF>
 __global__ void __launch_bounds__(kBlock) k_profile(Buffers B) {
-  const Brick g = B.bricks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const uint32_t gb = dealt_index(blockIdx.x);
+  if (gb >= B.n_bricks) return;
+  const Brick g = B.bricks[gb];
   const Replica R = B.rep[g.rep];
   const uint32_t bins = R.prof_bins;
   if (bins == 0) return;  // the whole workgroup leaves together
@@ -1060,6 +1227,63 @@ __global__ void __launch_bounds__(kBlock) k_profile(Buffers B) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Heat current (lennard_jones.wgsl:872-947, R14): J = sum_i e_i v_i +
+// 1/2 sum_i sum_j (F_ij . v_i) r_ij, e_i the kinetic energy plus half of
+// each pair energy; per atom over its list in order, each component to
+// fixed point at 2^32, one 64-bit atomic per workgroup and component into
+// sample `sample` of the series. Pair terms in FP64.
+// ---------------------------------------------------------------------------
+
+template <bool SF>
+__global__ void __launch_bounds__(kBlock) k_heat_current(Buffers B, uint32_t sample) {
+  const uint32_t gb = dealt_index(blockIdx.x);
+  if (gb >= B.n_bricks) return;
+  const Brick g = B.bricks[gb];
+  const Replica R = B.rep[g.rep];
+  if (R.hc_cap == 0u || sample >= R.hc_cap) return;
+  __shared__ Stage S;
+  const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+  long long jsum[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t H = S.hoff[g.gx * g.gy];
+  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+    const Home at = home_atom(g, S, h);
+    const uint32_t i = at.slot;
+    double xi, yi, zi;
+    staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
+    const double v[3] = {B.vx[i], B.vy[i], B.vz[i]};
+    double e = 0.5 * dist2(v[0], v[1], v[2]);
+    double pr[3] = {0.0, 0.0, 0.0};
+    const uint32_t cnt = B.nbr_count[i];
+    const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
+    for (uint32_t k = 0; k < cnt; k++) {
+      double xj, yj, zj;
+      staged_pos(R, S, staged, lst[list_entry(k)], B.x, B.y, B.z, xj, yj, zj);
+      const double d[3] = {xj - xi, yj - yi, zj - zi};
+      const double r2 = dist2(d[0], d[1], d[2]);
+      if (!(r2 < R.rc2)) continue;
+      const double inv2 = 1.0 / r2;
+      const double s6 = inv2 * inv2 * inv2;
+      double rf = s6 * __builtin_fma(48.0, s6, -24.0);
+      const double t = 4.0 * s6;
+      double u = __builtin_fma(t, s6, -t) - R.shift;
+      if (SF) {
+        const double r = __builtin_sqrt(r2);
+        rf = __builtin_fma(-r, R.force_shift, rf);
+        u = __builtin_fma(r - R.rc, R.force_shift, u);
+      }
+      e += 0.5 * u;
+      const double dv = __builtin_fma(d[2], v[2], __builtin_fma(d[1], v[1], d[0] * v[0]));
+      const double q = dv * rf / r2;
+#pragma unroll
+      for (int a = 0; a < 3; a++) pr[a] = __builtin_fma(d[a], q, pr[a]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) jsum[a] += to_fixed(__builtin_fma(e, v[a], 0.5 * pr[a]), 4294967296.0);
+  }
+  block_add_slots<3>(jsum, B.hc + 3ull * sample);
+}
+
 __global__ void __launch_bounds__(kBlock) k_zero_slots(Buffers B) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < B.n_replicas * kSlots) B.slots[t] = 0;
@@ -1110,29 +1334,46 @@ hipError_t launch_decide(const Buffers& B, hipStream_t s) {
 hipError_t launch_drift(const Buffers& B, bool langevin, hipStream_t s) {
   if (!B.n_atom_blocks) return hipSuccess;
   if (langevin)
-    hipLaunchKernelGGL(k_drift<true>, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL(k_drift<true>, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
   else
-    hipLaunchKernelGGL(k_drift<false>, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL(k_drift<false>, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_decide, decide_grid(B), dim3(kBlock), 0, s, B, 1);
   return hipGetLastError();
 }
 
 hipError_t launch_nh_half(const Buffers& B, bool scale, hipStream_t s) {
   if (!B.n_atom_blocks) return hipSuccess;
-  hipLaunchKernelGGL(k_ke_sum, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_ke_sum, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_nh_update, decide_grid(B), dim3(kBlock), 0, s, B);
-  if (scale) hipLaunchKernelGGL(k_nh_scale, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+  if (scale) hipLaunchKernelGGL(k_nh_scale, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_heat_exchange(const Buffers& B, hipStream_t s) {
+  if (!B.n_atom_blocks) return hipSuccess;
+  hipLaunchKernelGGL(k_hex_sum, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_hex_update, decide_grid(B), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_hex_apply, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, hipStream_t s) {
+  if (!B.n_bricks) return hipSuccess;
+  if (shifted)
+    hipLaunchKernelGGL((k_heat_current<true>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B, sample);
+  else
+    hipLaunchKernelGGL((k_heat_current<false>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B, sample);
   return hipGetLastError();
 }
 
 hipError_t launch_rebuild(const Buffers& B, hipStream_t s) {
   if (!B.n_atom_blocks) return hipSuccess;
-  hipLaunchKernelGGL(k_bin, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_bin, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_scan, grid1(B.n_scan_blocks), dim3(kBlock), 0, s, B);
-  hipLaunchKernelGGL(k_scatter, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
-  hipLaunchKernelGGL(k_sort, grid1(B.n_sort_blocks), dim3(kBlock), 0, s, B);
-  hipLaunchKernelGGL(k_reorder, grid1(B.n_atom_blocks), dim3(kBlock), 0, s, B);
-  hipLaunchKernelGGL(k_build, grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_scatter, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_sort, grid1(dealt_grid(B.n_sort_blocks)), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_reorder, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_build, grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
   return hipGetLastError();
 }
 
@@ -1142,14 +1383,14 @@ hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick
   const int k = kick ? 1 : 0;
   if (precision == 0) {
     if (shifted)
-      hipLaunchKernelGGL((k_force<0, true>), grid1(nb), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<0, true>), grid1(dealt_grid(nb)), dim3(kBlock), 0, s, B, k);
     else
-      hipLaunchKernelGGL((k_force<0, false>), grid1(nb), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<0, false>), grid1(dealt_grid(nb)), dim3(kBlock), 0, s, B, k);
   } else {
     if (shifted)
-      hipLaunchKernelGGL((k_force<1, true>), grid1(nb), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<1, true>), grid1(dealt_grid(nb)), dim3(kBlock), 0, s, B, k);
     else
-      hipLaunchKernelGGL((k_force<1, false>), grid1(nb), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<1, false>), grid1(dealt_grid(nb)), dim3(kBlock), 0, s, B, k);
   }
   return hipGetLastError();
 }
@@ -1160,14 +1401,14 @@ hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStre
   if (!B.n_atom_blocks) return hipGetLastError();
   if (precision == 0) {
     if (shifted)
-      hipLaunchKernelGGL((k_measure<0, true>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<0, true>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
     else
-      hipLaunchKernelGGL((k_measure<0, false>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<0, false>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
   } else {
     if (shifted)
-      hipLaunchKernelGGL((k_measure<1, true>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<1, true>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
     else
-      hipLaunchKernelGGL((k_measure<1, false>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<1, false>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
   }
   return hipGetLastError();
 }
@@ -1175,9 +1416,9 @@ hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStre
 hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s) {
   if (!B.n_bricks) return hipSuccess;
   if (shifted)
-    hipLaunchKernelGGL((k_profile<true>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL((k_profile<true>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
   else
-    hipLaunchKernelGGL((k_profile<false>), grid1(B.n_bricks), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL((k_profile<false>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
   return hipGetLastError();
 }
 

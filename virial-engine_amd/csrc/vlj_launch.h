@@ -13,6 +13,10 @@ hipError_t launch_rebuild(const Buffers& B, hipStream_t s);
 // Nose-Hoover half step: kinetic energy, xi / eta / scale; with `scale` the
 // velocities are scaled too (the second half step of a step).
 hipError_t launch_nh_half(const Buffers& B, bool scale, hipStream_t s);
+// Heat exchange after the step (sums, scales, apply).
+hipError_t launch_heat_exchange(const Buffers& B, hipStream_t s);
+// One heat-current sample into sample `sample` of the series.
+hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, hipStream_t s);
 hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s);
 hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStream_t s);
 hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s);

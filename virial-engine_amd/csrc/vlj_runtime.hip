@@ -39,7 +39,7 @@ struct Ring {
 struct HostReplica {
   vlj_settings settings;
   double energy_scale, momentum_scale;
-  uint32_t brick_x, brick_y;  // brick shape in cells
+  std::vector<uint32_t> brick_x, brick_y;  // brick shape in cells, per z layer
   uint64_t prof_samples;      // profile samples since the last clear
 };
 
@@ -60,6 +60,8 @@ struct vlj_sim {
   bool shifted = false;
   bool any_langevin = false;
   bool any_nose_hoover = false;
+  bool any_heat_exchange = false;
+  uint32_t hc_samples = 0;  // heat-current samples recorded (replica 0)
   Buffers B{};
   std::vector<void*> allocations;
   Ring ring[kRing];
@@ -262,6 +264,8 @@ void free_all(vlj_sim* s) {
   }
   for (void* p : s->allocations) (void)hipFree(p);
   s->allocations.clear();
+  if (s->B.hc) (void)hipFree(s->B.hc);
+  s->B.hc = nullptr;
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
 }
@@ -290,7 +294,17 @@ int enqueue_step(vlj_sim* s) {
   HIP_TRY(s, launch_rebuild(s->B, s->stream));
   HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, true, s->stream));
   if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, true, s->stream));
+  if (s->any_heat_exchange) HIP_TRY(s, launch_heat_exchange(s->B, s->stream));
   return VLJ_OK;
+}
+
+// The captured steps change shape (a kernel more or less): capture again
+// on the next step.
+void drop_graphs(vlj_sim* s) {
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->graph1) (void)hipGraphExecDestroy(s->graph1);
+  if (s->graphN) (void)hipGraphExecDestroy(s->graphN);
+  s->graph1 = s->graphN = nullptr;
 }
 
 int capture(vlj_sim* s, uint32_t steps, hipGraphExec_t* out) {
@@ -326,36 +340,66 @@ int prime_if_needed(vlj_sim* s) {
 // initial cell occupancy, holds at most 85% of kStageMax atoms; the margin
 // absorbs density fluctuations. A brick that still overflows runs a slower
 // path through global memory.
-void choose_brick(const Replica& R, const std::vector<uint32_t>& occ, uint32_t& bx, uint32_t& by) {
+// Brick shape of each z layer: the largest gx x gy (home cells, up to
+// kBrickMax each way) whose every brick stages at most 85% of kStageMax
+// atoms. The count a brick is held to is the larger of what it stages in
+// the initial state at its own layer and at the layers above and below (a
+// density profile may drift by a cell, as a liquid-vapour interface does)
+// and 1.1 times the replica's mean occupancy (a lattice that is not yet
+// uniform evens out). Window sums come from a 2D prefix sum of the three
+// layers' counts over the periodically extended plane.
+void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vector<uint32_t>& bx,
+                   std::vector<uint32_t>& by) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
   const double limit = 0.85 * kStageMax;
-  bx = 1;
-  by = 1;
-  uint32_t best = 0;
-  for (int gy = 1; gy <= std::min(kBrickMax, ny); gy++)
-    for (int gx = gy; gx <= std::min(kBrickMax, nx); gx++) {
-      const uint32_t home = (uint32_t)(gx * gy);
-      if (home <= best) continue;
-      uint32_t worst = 0;
-      for (int z = 0; z < nz && worst <= limit; z++)
-        for (int y0 = 0; y0 < ny && worst <= limit; y0 += gy)
-          for (int x0 = 0; x0 < nx && worst <= limit; x0 += gx) {
-            uint32_t sum = 0;
-            const int ex = std::min(gx, nx - x0), ey = std::min(gy, ny - y0);
-            for (int dz = -1; dz <= 1; dz++)
-              for (int yy = -1; yy <= ey; yy++)
-                for (int xx = -1; xx <= ex; xx++) {
-                  const int cx = (x0 + xx + nx) % nx, cy = (y0 + yy + ny) % ny, cz = (z + dz + nz) % nz;
-                  sum += occ[(size_t)(cz * ny + cy) * nx + cx];
-                }
-            worst = std::max(worst, sum);
-          }
-      if (worst <= limit) {
-        best = home;
-        bx = (uint32_t)gx;
-        by = (uint32_t)gy;
+  const double mean = (double)R.count / (double)R.total_cells;
+  const int ex = nx + 2, ey = ny + 2;  // plane extended by one cell each side
+  const size_t plane = (size_t)(ex + 1) * (ey + 1);
+  std::vector<uint64_t> pre(plane * nz);
+  for (int z = 0; z < nz; z++) {
+    uint64_t* P = pre.data() + plane * z;
+    for (int y = 0; y <= ey; y++)
+      for (int x = 0; x <= ex; x++) {
+        uint64_t v = 0;
+        if (x > 0 && y > 0) {
+          const int cx = (x - 2 + nx) % nx, cy = (y - 2 + ny) % ny;  // extended x - 1 is cell x - 2
+          for (int dz = -1; dz <= 1; dz++) v += occ[(size_t)(((z + dz + nz) % nz) * ny + cy) * nx + cx];
+          v += P[(size_t)y * (ex + 1) + (x - 1)] + P[(size_t)(y - 1) * (ex + 1) + x] -
+               P[(size_t)(y - 1) * (ex + 1) + (x - 1)];
+        }
+        P[(size_t)y * (ex + 1) + x] = v;
       }
-    }
+  }
+  // staged cells of the brick at (x0, y0): x0 - 1 .. x0 + w, y0 - 1 .. y0 + h,
+  // i.e. extended columns x0 .. x0 + w + 1
+  auto window = [&](int z, int x0, int y0, int w, int h) {
+    const uint64_t* P = pre.data() + plane * z;
+    const int xa = x0, xb = x0 + w + 2, ya = y0, yb = y0 + h + 2;
+    return P[(size_t)yb * (ex + 1) + xb] - P[(size_t)ya * (ex + 1) + xb] - P[(size_t)yb * (ex + 1) + xa] +
+           P[(size_t)ya * (ex + 1) + xa];
+  };
+  bx.assign(nz, 1);
+  by.assign(nz, 1);
+  for (int z = 0; z < nz; z++) {
+    uint32_t best = 0;
+    for (int gy = 1; gy <= std::min(kBrickMax, ny); gy++)
+      for (int gx = gy; gx <= std::min(kBrickMax, nx); gx++) {
+        const uint32_t home = (uint32_t)(gx * gy);
+        if (home <= best) continue;
+        double worst = 1.1 * mean * 3.0 * (gx + 2) * (gy + 2);
+        for (int dz = -1; dz <= 1 && worst <= limit; dz++) {
+          const int zz = (z + dz + nz) % nz;
+          for (int y0 = 0; y0 < ny && worst <= limit; y0 += gy)
+            for (int x0 = 0; x0 < nx && worst <= limit; x0 += gx)
+              worst = std::max(worst, (double)window(zz, x0, y0, std::min(gx, nx - x0), std::min(gy, ny - y0)));
+        }
+        if (worst <= limit) {
+          best = home;
+          bx[z] = (uint32_t)gx;
+          by[z] = (uint32_t)gy;
+        }
+      }
+  }
 }
 
 // Builds a handle over `replicas` systems.
@@ -409,11 +453,10 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     }
     R.atom_block_off = (uint32_t)s->atom_blocks.size();
     R.atom_blocks = (R.count + kBlock - 1) / kBlock;
-    choose_brick(R, occ, s->host[r].brick_x, s->host[r].brick_y);
-    {
-      const uint32_t gx = s->host[r].brick_x, gy = s->host[r].brick_y;
-      for (uint32_t z = 0; z < R.cells[2]; z++)
-        for (uint32_t y0 = 0; y0 < R.cells[1]; y0 += gy)
+    choose_bricks(R, occ, s->host[r].brick_x, s->host[r].brick_y);
+    for (uint32_t z = 0; z < R.cells[2]; z++) {
+      const uint32_t gx = s->host[r].brick_x[z], gy = s->host[r].brick_y[z];
+      for (uint32_t y0 = 0; y0 < R.cells[1]; y0 += gy)
           for (uint32_t x0 = 0; x0 < R.cells[0]; x0 += gx)
             s->bricks.push_back({(uint32_t)r, (z * R.cells[1] + y0) * R.cells[0] + x0,
                                  std::min(gx, R.cells[0] - x0), std::min(gy, R.cells[1] - y0)});
@@ -470,6 +513,8 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   A(&gb, s->bricks.size());
   A(&B.bmax, s->atom_blocks.size());
   A(&B.kepart, s->atom_blocks.size());
+  A(&B.hexpart, s->atom_blocks.size() * kHexSums);
+  A(&B.hexstate, replicas * kHexState);
   double** d[] = {&B.x, &B.y, &B.z, &B.vx, &B.vy, &B.vz, &B.ax, &B.ay, &B.az,
                   &B.x2, &B.y2, &B.z2, &B.vx2, &B.vy2, &B.vz2, &B.ax2, &B.ay2, &B.az2,
                   &B.xb, &B.yb, &B.zb};
@@ -883,6 +928,106 @@ int vlj_batch_read_profile_raw(vlj_batch* sim, size_t replica, int64_t* raw_out,
   return rc != VLJ_OK ? rc : copy_raw(sim, raw, raw_out, capacity);
 }
 
+// lennard_jones.rs:927-943: constant energy only; the captured step gains
+// or loses the exchange kernels.
+int vlj_set_heat_exchange(vlj_sim* sim, int32_t on, const double hot[2], const double cold[2], double energy) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  Replica& R = sim->rep[0];
+  if (R.thermostat != 0u)
+    return fail(sim, VLJ_E_STATE, "heat exchange runs at constant energy, without a thermostat");
+  if (on && (!hot || !cold || !std::isfinite(energy)))
+    return fail(sim, VLJ_E_INVALID, "heat exchange needs hot and cold slabs and a finite energy");
+  R.hex_on = on ? 1u : 0u;
+  for (int k = 0; k < 2; k++) {
+    R.hex_slabs[k] = on ? hot[k] : 0.0;
+    R.hex_slabs[2 + k] = on ? cold[k] : 0.0;
+  }
+  R.hex_energy = on ? energy : 0.0;
+  bool any = false;
+  for (const Replica& r : sim->rep) any = any || r.hex_on;
+  HIP_TRY(sim, hipMemcpyAsync((void*)&sim->B.rep[0], &R, sizeof(Replica), hipMemcpyHostToDevice, sim->stream));
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  if (any != sim->any_heat_exchange) {
+    drop_graphs(sim);
+    sim->any_heat_exchange = any;
+  }
+  return VLJ_OK;
+}
+
+// lennard_jones.rs:945-951: energy moved per step over 2 A dt.
+int vlj_heat_flux(const vlj_sim* sim, double* flux, int32_t* has) {
+  if (!sim || !flux || !has) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_heat_flux");
+  const Replica& R = sim->rep[0];
+  *has = R.hex_on ? 1 : 0;
+  *flux = R.hex_on ? R.hex_energy / (2.0 * R.box[0] * R.box[1] * R.dt) : 0.0;
+  return VLJ_OK;
+}
+
+// lennard_jones.rs:953-973: room for `capacity` samples, any recorded dropped.
+int vlj_record_heat_current(vlj_sim* sim, uint32_t capacity) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  if (sim->B.hc) HIP_TRY(sim, hipFree(sim->B.hc));
+  sim->B.hc = nullptr;
+  if (capacity) {
+    const size_t words = (size_t)capacity * 3;
+    HIP_TRY(sim, hipMalloc((void**)&sim->B.hc, words * sizeof(long long)));
+    HIP_TRY(sim, launch_zero(sim->B.hc, (uint32_t)words, sim->stream));
+  }
+  sim->rep[0].hc_cap = capacity;
+  sim->hc_samples = 0;
+  HIP_TRY(sim, hipMemcpyAsync((void*)&sim->B.rep[0], &sim->rep[0], sizeof(Replica), hipMemcpyHostToDevice,
+                              sim->stream));
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  drop_graphs(sim);  // the graphs hold the old buffer pointers
+  return VLJ_OK;
+}
+
+// lennard_jones.rs:975-996: one sample, asynchronous.
+int vlj_sample_heat_current(vlj_sim* sim) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (!sim->B.hc) return fail(sim, VLJ_E_STATE, "record_heat_current first");
+  if (sim->hc_samples >= sim->rep[0].hc_cap) return fail(sim, VLJ_E_STATE, "heat current series full");
+  int rc = prime_if_needed(sim);
+  if (rc != VLJ_OK) return rc;
+  HIP_TRY(sim, launch_heat_current(sim->B, sim->shifted, sim->hc_samples, sim->stream));
+  sim->hc_samples++;
+  return VLJ_OK;
+}
+
+static int read_series(vlj_sim* sim, std::vector<long long>& raw, uint32_t* samples) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (!sim->B.hc) return fail(sim, VLJ_E_STATE, "record_heat_current first");
+  raw.assign((size_t)sim->hc_samples * 3, 0);
+  if (!raw.empty()) {
+    int rc = read_dev(sim, raw.data(), sim->B.hc, raw.size() * sizeof(long long));
+    if (rc != VLJ_OK) return rc;
+  }
+  if (samples) *samples = sim->hc_samples;
+  return VLJ_OK;
+}
+
+// lennard_jones.rs:998-1012: the recorded samples, x y z each.
+int vlj_read_heat_current(vlj_sim* sim, double* out, size_t capacity, uint32_t* samples) {
+  std::vector<long long> raw;
+  int rc = read_series(sim, raw, samples);
+  if (rc != VLJ_OK) return rc;
+  if (raw.size() > 3 * capacity || (!out && !raw.empty()))
+    return fail(sim, VLJ_E_INVALID, "the output holds fewer samples than recorded");
+  for (size_t k = 0; k < raw.size(); k++) out[k] = (double)raw[k] / kTwo32;
+  return VLJ_OK;
+}
+
+int vlj_read_heat_current_raw(vlj_sim* sim, int64_t* out, size_t capacity, uint32_t* samples) {
+  std::vector<long long> raw;
+  int rc = read_series(sim, raw, samples);
+  if (rc != VLJ_OK) return rc;
+  if (raw.size() > 3 * capacity || (!out && !raw.empty()))
+    return fail(sim, VLJ_E_INVALID, "the output holds fewer samples than recorded");
+  if (!raw.empty()) std::memcpy(out, raw.data(), raw.size() * sizeof(long long));
+  return VLJ_OK;
+}
+
 const char* vlj_last_error(const vlj_sim* sim) { return sim ? sim->error.c_str() : g_error.c_str(); }
 
 int vlj_batch_create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings* settings,
@@ -988,10 +1133,10 @@ int vlj_debug_lists(vlj_sim* sim, uint32_t* counts, uint32_t* entries) {
     if (rc == VLJ_OK) rc = read_dev(sim, sorted.data(), B.sorted + R.atom_off, (size_t)R.count * 4);
     if (rc != VLJ_OK) return rc;
     const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
-    const int bx = (int)sim->host[0].brick_x, by = (int)sim->host[0].brick_y;
     for (uint32_t i = 0; i < R.count; i++) {
       const int c = (int)(cell_of[i] - R.cell_off);
       const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+      const int bx = (int)sim->host[0].brick_x[cz], by = (int)sim->host[0].brick_y[cz];
       const int x0 = cx / bx * bx, y0 = cy / by * by;
       const int gx = std::min(bx, nx - x0), gy = std::min(by, ny - y0);
       std::vector<uint32_t> off, first;  // staged offset and first slot of each staged cell
@@ -1036,6 +1181,21 @@ int vlj_debug_grid(const vlj_sim* sim, uint32_t out[6]) {
   return VLJ_OK;
 }
 
+int vlj_debug_counters(const vlj_sim* sim, uint64_t out[4]) {
+  if (!sim || !out) return VLJ_E_INVALID;
+  std::vector<Control> c(sim->rep.size());
+  if (!c.empty() && hipMemcpy(c.data(), sim->B.ctl, c.size() * sizeof(Control), hipMemcpyDeviceToHost) != hipSuccess)
+    return VLJ_E_HIP;
+  out[0] = out[1] = out[2] = 0;
+  for (const Control& x : c) {
+    out[0] += x.rebuilds;
+    out[1] += x.stage_overflow;
+    out[2] = std::max<uint64_t>(out[2], x.overflow);
+  }
+  out[3] = sim->bricks.size();
+  return VLJ_OK;
+}
+
 int vlj_debug_params(const vlj_sim* sim, double out[6]) {
   if (!sim || !out) return VLJ_E_INVALID;
   out[0] = sim->host[0].energy_scale;
@@ -1047,7 +1207,9 @@ int vlj_debug_params(const vlj_sim* sim, double out[6]) {
   return VLJ_OK;
 }
 
-uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks) { return xcd_swizzle(block, blocks); }
+uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks) {
+  return blocks == 0 ? dealt_grid(block) : dealt_index(block);
+}
 
 int vlj_copy_bandwidth(size_t bytes, uint32_t repeats, double* ms) {
   if (!ms || bytes < 16 || repeats == 0) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_copy_bandwidth");

@@ -101,6 +101,12 @@ SIGNATURES = {
     "vlj_read_profile": (C.c_int, [_P, _dp, _sz, _u64p]),
     "vlj_read_profile_raw": (C.c_int, [_P, _i64p, _sz, _u64p]),
     "vlj_batch_read_profile_raw": (C.c_int, [_P, _sz, _i64p, _sz, _u64p]),
+    "vlj_set_heat_exchange": (C.c_int, [_P, _i32, _dp, _dp, _d]),
+    "vlj_heat_flux": (C.c_int, [_P, _dp, C.POINTER(_i32)]),
+    "vlj_record_heat_current": (C.c_int, [_P, _u32]),
+    "vlj_sample_heat_current": (C.c_int, [_P]),
+    "vlj_read_heat_current": (C.c_int, [_P, _dp, _sz, _up]),
+    "vlj_read_heat_current_raw": (C.c_int, [_P, _i64p, _sz, _up]),
     "vlj_last_error": (C.c_char_p, [_P]),
     "vlj_batch_create": (C.c_int, [_pp, _u64p, C.POINTER(Settings), _sz, C.POINTER(_P)]),
     "vlj_batch_run": (C.c_int, [_P, _u64]),
@@ -118,6 +124,7 @@ SIGNATURES = {
     "vlj_debug_lists": (C.c_int, [_P, _up, _up]),
     "vlj_debug_grid": (C.c_int, [_P, _up]),
     "vlj_debug_params": (C.c_int, [_P, _dp]),
+    "vlj_debug_counters": (C.c_int, [_P, _u64p]),
     "vlj_debug_swizzle": (_u32, [_u32, _u32]),
     "vlj_copy_bandwidth": (C.c_int, [_sz, _u32, _dp]),
     "vlj_debug_rcp": (C.c_int, [_dp, _dp, _sz]),

@@ -68,6 +68,16 @@ Thermostat = Optional[Union[Langevin, NoseHoover]]
 
 
 @dataclass
+class HeatExchange:
+    """Heat exchange between a hot and a cold slab along z
+    (lennard_jones.rs:376-387): `energy` moved per step."""
+
+    hot: Sequence[float]
+    cold: Sequence[float]
+    energy: float
+
+
+@dataclass
 class Settings:
     """Box, cutoff, time step and thermostat (lennard_jones.rs:63-82), plus
     the pair-force precision (FP64, or FP32 on FP64 displacements)."""
@@ -418,6 +428,41 @@ class LennardJones:
     def clear_profile(self) -> None:
         self._c(L.lib().vlj_clear_profile(self._h))
 
+    # ---- heat exchange and heat current (lennard_jones.rs:927-1012)
+    def set_heat_exchange(self, exchange: Optional[HeatExchange]) -> None:
+        on = exchange is not None
+        hot = np.ascontiguousarray(exchange.hot if on else [0.0, 0.0], dtype=np.float64)
+        cold = np.ascontiguousarray(exchange.cold if on else [0.0, 0.0], dtype=np.float64)
+        self._c(L.lib().vlj_set_heat_exchange(self._h, int(on), _dptr(hot), _dptr(cold),
+                                              float(exchange.energy) if on else 0.0))
+
+    def heat_flux(self) -> Optional[float]:
+        flux, has = C.c_double(), C.c_int32()
+        self._c(L.lib().vlj_heat_flux(self._h, C.byref(flux), C.byref(has)))
+        return flux.value if has.value else None
+
+    def record_heat_current(self, capacity: int) -> None:
+        self._hc_capacity = int(capacity)
+        self._c(L.lib().vlj_record_heat_current(self._h, self._hc_capacity))
+
+    def sample_heat_current(self) -> None:
+        self._c(L.lib().vlj_sample_heat_current(self._h))
+
+    def read_heat_current(self) -> List[List[float]]:
+        cap = max(getattr(self, "_hc_capacity", 0), 1)
+        out = np.zeros(3 * cap)
+        n = C.c_uint32()
+        self._c(L.lib().vlj_read_heat_current(self._h, _dptr(out), cap, C.byref(n)))
+        return out[: 3 * n.value].reshape(-1, 3).tolist()
+
+    def read_heat_current_raw(self) -> np.ndarray:
+        cap = max(getattr(self, "_hc_capacity", 0), 1)
+        out = np.zeros((cap, 3), dtype=np.int64)
+        n = C.c_uint32()
+        self._c(L.lib().vlj_read_heat_current_raw(self._h, out.ctypes.data_as(C.POINTER(C.c_int64)), cap,
+                                                  C.byref(n)))
+        return out[: n.value]
+
     def _profile_size(self) -> int:
         return int(self._settings.profile_bins) * PROFILE_SUMS + 3
 
@@ -450,6 +495,9 @@ class LennardJones:
         return {"cells": [int(x) for x in out[:3]], "total_cells": int(out[3]), "capacity": int(out[4]),
                 "rebuilds": int(out[5])}
 
+    def debug_counters(self) -> dict:
+        return _debug_counters(self._h)
+
     def debug_params(self) -> dict:
         out = np.zeros(6)
         self._c(L.lib().vlj_debug_params(self._h, _dptr(out)))
@@ -478,6 +526,13 @@ class LennardJones:
         f, t = C.c_double(), C.c_double()
         self._c(L.lib().vlj_timed_run(self._h, steps, C.byref(f), C.byref(t)))
         return f.value, t.value
+
+
+def _debug_counters(h) -> dict:
+    out = np.zeros(4, dtype=np.uint64)
+    _check(L.lib().vlj_debug_counters(h, out.ctypes.data_as(C.POINTER(C.c_uint64))), h)
+    return {"rebuilds": int(out[0]), "stage_overflow_builds": int(out[1]), "list_overflow": int(out[2]),
+            "bricks": int(out[3])}
 
 
 class Batch:
@@ -524,6 +579,9 @@ class Batch:
 
     def sample_profile(self) -> None:
         _check(L.lib().vlj_sample_profile(self._h), self._h)
+
+    def debug_counters(self) -> dict:
+        return _debug_counters(self._h)
 
     def read_profile_raw(self, replica: int):
         out = np.zeros(self.profile_bins[replica] * PROFILE_SUMS + 3, dtype=np.int64)
