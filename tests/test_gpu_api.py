@@ -145,7 +145,7 @@ def test_dense_cluster_takes_the_global_memory_path(oracle, lj):
     # LDS and read positions from global memory; results stay exact
     rng = np.random.default_rng(2)
     box = 40.0  # rc 3.5: 10 cells of 4.0 per axis
-    blob, _ = oracle.lattice(8000, 17.0, 1.0, 4)  # spacing 0.85: about 104 atoms per filled cell
+    blob, _ = oracle.lattice(8000, 15.0, 1.0, 4)  # spacing 0.75: about 150 atoms per filled cell
     blob = blob + 11.5
     gas = rng.uniform(0, box, (300, 3))
     d = gas[:, None, :] - blob[None, :, :]
@@ -162,7 +162,8 @@ def test_dense_cluster_takes_the_global_memory_path(oracle, lj):
     # some 3 x 3 x 3 neighbourhood holds more atoms than a brick can stage
     counts = np.diff(ref.cells()[1].astype(np.int64)).reshape(10, 10, 10)
     hood = sum(np.roll(counts, (a, b, c), axis=(0, 1, 2)) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1))
-    assert hood.max() > 2048
+    assert hood.max() > 3200
+    assert sim.debug_counters()["stage_overflow_builds"] > 0
 
 
 def test_two_cells_per_axis(oracle, lj):

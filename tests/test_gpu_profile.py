@@ -97,7 +97,7 @@ def test_profile_global_memory_staging(oracle, lj):
     # a dense blob whose bricks cannot stage their neighbourhood in LDS
     rng = np.random.default_rng(2)
     box = 40.0
-    blob, _ = oracle.lattice(8000, 17.0, 1.0, 4)
+    blob, _ = oracle.lattice(8000, 15.0, 1.0, 4)
     blob = blob + 11.5
     gas = rng.uniform(0, box, (300, 3))
     d = gas[:, None, :] - blob[None, :, :]
@@ -110,6 +110,7 @@ def test_profile_global_memory_staging(oracle, lj):
     sim.sample_profile()
     ref.sample_profile()
     np.testing.assert_array_equal(sim.read_profile_raw()[0], ref.read_profile()[0])
+    assert sim.debug_counters()["stage_overflow_builds"] > 0
 
 
 def test_batch_profiles_match_single_runs(oracle, lj):

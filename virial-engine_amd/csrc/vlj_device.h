@@ -18,6 +18,11 @@
 namespace vlj {
 
 constexpr int kBlock = 256;       // threads per atom block (4 wave64)
+#ifndef VLJ_BRICK_THREADS
+#define VLJ_BRICK_THREADS 512
+#endif
+constexpr int kBrickThreads = VLJ_BRICK_THREADS;  // threads of a brick workgroup
+static_assert(kBrickThreads % 64 == 0 && kBrickThreads <= 1024, "whole waves");
 constexpr int kScanCells = 1024;  // cells per scan block (256 threads x 4)
 constexpr int kSortCells = 4;     // cells per sort block (one wave64 per cell)
 constexpr uint32_t kProfileSums = 8;       // sums per profile slab (lennard_jones.wgsl:644)
@@ -27,8 +32,19 @@ constexpr uint32_t kHexState = 18;  // per slab: com x3, mean force x3, s - 1, c
 constexpr int kBrickMax = 8;      // home cells of a brick along x and along y, at most
 constexpr int kStageCells = 3 * (kBrickMax + 2) * (kBrickMax + 2);  // staged cells, at most
 constexpr int kHomeCells = kBrickMax * kBrickMax;                    // home cells, at most
-static_assert(kStageCells <= 2 * kBlock, "staging takes two cells per thread");
-constexpr int kStageMax = 2048;   // atoms a brick workgroup stages in LDS
+static_assert(kHomeCells <= 64, "one lane of a wave per home cell");
+static_assert(kStageCells <= 2 * kBrickThreads, "staging takes two cells per thread");
+#ifndef VLJ_STAGE_MAX
+#define VLJ_STAGE_MAX 3200
+#endif
+// Atoms a brick workgroup stages in LDS. With the sentinel the staged arrays
+// hold kStageMax + 64 doubles each (a multiple of 64, for ds_read2st64); at
+// 3200 the stage takes ~79 KB, two 512-thread workgroups per CU (16 waves).
+// Measured on the 1M-atom liquid: 256 threads / 2048 atoms (3 workgroups
+// per CU, bricks of ~170 atoms leaving a wave in four idle) 0.359 ms per
+// step; 512 / 3008: 0.338; 512 / 3200 with bricks planned to 90%: 0.332.
+constexpr int kStageMax = VLJ_STAGE_MAX;
+static_assert(kStageMax % 64 == 0, "staged arrays are whole rows of 64");
 constexpr int kListStride = 64;   // lists are transposed per block of 64 atoms (one wave)
 constexpr int kSlots = 8;         // fixed-point slots per replica
 constexpr double kNearR2 = 2.25;  // condensed: neighbours within 1.5 sigma (lennard_jones.wgsl:417)

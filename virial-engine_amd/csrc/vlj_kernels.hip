@@ -587,7 +587,7 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
   const int c0 = (int)g.cell0;
   const int cx0 = c0 % nx, cy0 = (c0 / nx) % ny, cz = c0 / (nx * ny);
   const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
-  const uint32_t ns = 3u * sy * sx;  // at most kStageCells <= 2 kBlock
+  const uint32_t ns = 3u * sy * sx;  // at most kStageCells <= 2 kBrickThreads
   const uint32_t t = threadIdx.x;
   // thread t stages cells 2t and 2t + 1
   uint32_t n_c[2] = {0u, 0u};
@@ -609,7 +609,7 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
     }
   }
   // exclusive scan of the staged counts over the workgroup
-  __shared__ uint32_t wtot[kBlock / 64];
+  __shared__ uint32_t wtot[kBrickThreads / 64];
   const uint32_t n_t = n_c[0] + n_c[1];
   const int lane = t & 63, wave = t >> 6;
   const uint32_t incl = wave_incl_scan_u32(n_t, lane);
@@ -620,31 +620,33 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
   const uint32_t first = before + incl - n_t;
   if (2u * t < ns) S.off[2u * t] = first;
   if (2u * t + 1u < ns) S.off[2u * t + 1u] = first + n_c[0];
-  if (t == kBlock - 1) {
+  if (t == kBrickThreads - 1) {
     S.off[ns] = before + incl;
     S.total = before + incl;
     S.ns = ns;
   }
   __syncthreads();
-  if (t == 0) {  // home atoms, home cell by home cell (hy outer, hx inner)
-    uint32_t run = 0;
-    for (uint32_t h = 0; h < g.gx * g.gy; h++) {
-      S.hoff[h] = run;
-      const uint32_t hs = staged_cell(g, 1u, h / g.gx + 1u, h % g.gx + 1u);
-      run += S.off[hs + 1u] - S.off[hs];
+  if (t < 64u) {  // home atoms, home cell by home cell (hy outer, hx inner): a scan over wave 0
+    const uint32_t nh = g.gx * g.gy;  // at most kHomeCells = 64
+    uint32_t n = 0;
+    if (t < nh) {
+      const uint32_t hs = staged_cell(g, 1u, t / g.gx + 1u, t % g.gx + 1u);
+      n = S.off[hs + 1u] - S.off[hs];
     }
-    S.hoff[g.gx * g.gy] = run;
+    const uint32_t incl = wave_incl_scan_u32(n, lane);
+    if (t < nh) S.hoff[t] = incl - n;
+    if (t + 1u == nh) S.hoff[nh] = incl;
   }
   const uint32_t total = S.total;
   const bool staged = total <= (uint32_t)kStageMax;
   if (staged) {
     // every load of this thread's entries first, then the LDS stores
-    constexpr int kPer = kStageMax / kBlock;
+    constexpr int kPer = (kStageMax + kBrickThreads - 1) / kBrickThreads;
     uint32_t code[kPer];
     double px[kPer], py[kPer], pz[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
-      const uint32_t e = t + (uint32_t)k * kBlock;
+      const uint32_t e = t + (uint32_t)k * kBrickThreads;
       code[k] = 13u;
       px[k] = py[k] = pz[k] = 0.0;
       if (e < total) {
@@ -658,7 +660,7 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
     }
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
-      const uint32_t e = t + (uint32_t)k * kBlock;
+      const uint32_t e = t + (uint32_t)k * kBrickThreads;
       if (e < total) {
         double shx, shy, shz;
         image_shift(R, code[k], shx, shy, shz);
@@ -727,20 +729,37 @@ __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// One workgroup per brick. The home atoms are cut into one tile per wave
-// (at most 64 atoms; more tiles when the brick has more than 256), and a
-// wave builds the lists of its tile together. For each home cell of the
-// tile and each of the nine runs (dz, dy) -- the three staged cells along x
-// of a run are staged one after the other, so a run is one range of staged
-// atoms in list order -- the 64 lanes take 64 candidates of the run at once.
-// Then, home atom by home atom, a ballot marks the candidates within
-// rc + skin (not the atom itself); each hit is stored at the atom's list
-// length plus its rank in the ballot, and the length grows by the ballot's
-// count. The lengths live one per lane (lane l: atom l of the tile). So the
-// entries come out in stencil order dz, dy, dx, then slot (R4).
+// A lane's value read from lane `l` (the same in every lane), as a scalar.
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+
+// Atoms of a home cell per work item of the list build, and chunks of 64
+// candidates a wave holds in registers at once.
+#ifndef VLJ_BUILD_PIECE
+#define VLJ_BUILD_PIECE 16
+#endif
+#ifndef VLJ_BUILD_GROUP
+#define VLJ_BUILD_GROUP 4
+#endif
+constexpr uint32_t kPiece = VLJ_BUILD_PIECE;
+constexpr int kGroup = VLJ_BUILD_GROUP;
+static_assert(kPiece >= 1 && kPiece <= 64, "a lane per home atom of a work item");
+
+// One workgroup per brick; its waves build the lists of one work item (a
+// piece of a home cell's atoms) at a time. A home cell's candidates are the
+// atoms of its 27 neighbour cells in list order: the nine runs (dz, dy) one
+// after the other, each run the three staged cells along x, which are
+// staged one after the other, so a run is one range of staged atoms. The
+// 64 lanes take 64 consecutive candidates each (up to kGroup such chunks in
+// registers). Then, home atom by home atom, a ballot per chunk marks the
+// candidates within rc + skin (not the atom itself); each hit is stored at
+// the atom's list length plus its rank in the ballot, and the length grows
+// by the ballot's count. So the entries come out in stencil order dz, dy,
+// dx, then slot (R4), and a sparse cell's 27 neighbours share chunks.
 // On a reorder the workgroup also copies its home atoms' state back from
 // the second set.
-__global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
+__global__ void __launch_bounds__(kBrickThreads) k_build(Buffers B) {
   const uint32_t gb = dealt_index(blockIdx.x);
   if (gb >= B.n_bricks) return;
   const Brick g = B.bricks[gb];
@@ -752,13 +771,15 @@ __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
   const double* Y = moved ? B.y2 : B.y;
   const double* Z = moved ? B.z2 : B.z;
   __shared__ Stage S;
-  const bool staged = stage_brick(B, R, g, X, Y, Z, S);
+  __shared__ uint32_t next_item;
+  if (threadIdx.x == 0) next_item = 0u;
+  const bool staged = stage_brick(B, R, g, X, Y, Z, S);  // ends with a barrier
   const uint32_t total = S.total;
   const uint32_t cap = R.capacity;
   if (!staged && threadIdx.x == 0) atomicAdd(&B.ctl[g.rep].stage_overflow, 1u);
   const uint32_t nh = g.gx * g.gy;
   const uint32_t H = S.hoff[nh];
-  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
     const Home a = home_atom(g, S, h);
     const uint32_t i = a.slot;
     double xi, yi, zi;
@@ -787,65 +808,105 @@ __global__ void __launch_bounds__(kBlock) k_build(Buffers B) {
     if (threadIdx.x == 0 && H > 0u && total > cap) atomicMax(&B.ctl[g.rep].overflow, total);
     return;
   }
-  const uint32_t lane = threadIdx.x & 63u, wave = uniform(threadIdx.x >> 6);
-  uint32_t tile = (uniform(H) + 3u) / 4u;
-  tile = tile < 1u ? 1u : (tile > 64u ? 64u : tile);
-  for (uint32_t t0 = wave * tile; t0 < H; t0 += 4u * tile) {
-    const uint32_t t1 = t0 + tile < H ? t0 + tile : H;
-    uint32_t found = 0;  // list length of home atom t0 + lane
-    uint32_t hc = 0;
-    while (hc + 1u < nh && S.hoff[hc + 1u] <= t0) hc++;
-    for (; hc < nh && S.hoff[hc] < t1; hc++) {
-      // values read from LDS are the same in every lane: keep them scalar
-      const uint32_t h0 = uniform(S.hoff[hc]), h1 = uniform(S.hoff[hc + 1u]);
-      const uint32_t k0 = t0 > h0 ? t0 : h0;
-      const uint32_t k1 = t1 < h1 ? t1 : h1;
-      if (k0 >= k1) continue;  // an empty home cell
-      const uint32_t hx = hc % g.gx, hy = hc / g.gx;
-      const uint32_t hs = staged_cell(g, 1u, hy + 1u, hx + 1u);
-      const uint32_t ebase = uniform(S.off[hs]) - h0;                 // staged index of home atom k: ebase + k
-      const uint32_t lbase = uniform(S.start[hs]) - R.atom_off - h0;  // its replica slot: lbase + k
-      for (uint32_t dz = 0; dz < 3u; dz++)
-        for (uint32_t dy = 0; dy < 3u; dy++) {
-          const uint32_t sc = staged_cell(g, dz, hy + dy, hx);
-          const uint32_t q1 = uniform(S.off[sc + 3u]);
-          for (uint32_t c0 = uniform(S.off[sc]); c0 < q1; c0 += 64u) {
-            const uint32_t q = c0 + lane;
-            const bool valid = q < q1;
-            double cx = kFar, cy = kFar, cz = kFar;
-            if (staged) {
-              const uint32_t qs = valid ? q : kListPad;
-              cx = S.x[qs];
-              cy = S.y[qs];
-              cz = S.z[qs];
-            } else if (valid) {
-              staged_pos(R, S, false, q, X, Y, Z, cx, cy, cz);
-            }
-            for (uint32_t k = k0; k < k1; k++) {
-              const uint32_t e = ebase + k;
-              double xi, yi, zi;
-              staged_pos(R, S, staged, e, X, Y, Z, xi, yi, zi);
-              const bool hit = q != e && dist2(cx - xi, cy - yi, cz - zi) < R.list_r2;
-              const unsigned long long mask = __builtin_amdgcn_ballot_w64(hit);
-              if (mask == 0ull) continue;
-              const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)found, (int)(k - t0));
+  // Work items: the atoms of each home cell cut into pieces of at most
+  // kPiece, taken by the waves one at a time from a counter in LDS. Lane c
+  // of every wave holds home cell c's first home atom, atom count, first
+  // item and item count.
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t hv = lane < nh ? S.hoff[lane] : H;
+  const uint32_t hn = lane < nh ? S.hoff[lane + 1u] - hv : 0u;
+  const uint32_t pieces = (hn + kPiece - 1u) / kPiece;
+  const uint32_t pincl = wave_incl_scan_u32(pieces, (int)lane);
+  const uint32_t pv = pincl - pieces;  // lanes past the home cells: the item count
+  const uint32_t n_items = readlane_u32(pincl, 63);
+  for (;;) {
+    uint32_t item = 0u;
+    if (lane == 0u) item = atomicAdd(&next_item, 1u);
+    item = readlane_u32(item, 0);
+    if (item >= n_items) break;
+    const uint32_t hc = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(pv <= item)) - 1u;
+    const uint32_t h0 = readlane_u32(hv, hc), n = readlane_u32(hn, hc);
+    const uint32_t p = item - readlane_u32(pv, hc), np = readlane_u32(pieces, hc);
+    const uint32_t k0 = h0 + p * n / np, k1 = h0 + (p + 1u) * n / np;  // home atoms k0 .. k1 - 1
+    const uint32_t hx = hc % g.gx, hy = hc / g.gx;
+    const uint32_t hs = staged_cell(g, 1u, hy + 1u, hx + 1u);
+    const uint32_t ebase = uniform(S.off[hs]) - h0;                 // staged index of home atom k: ebase + k
+    const uint32_t lbase = uniform(S.start[hs]) - R.atom_off - h0;  // its replica slot: lbase + k
+    // The candidates in list order: the nine runs (dz, dy), each the three
+    // staged cells hx .. hx + 2 of row hy + dy of layer dz, one after the
+    // other. Run r (lane r) holds sequence numbers rend[r - 1] .. rend[r] - 1;
+    // sequence number s of run r is staged atom s + rdelta[r].
+    uint32_t rs = 0u, rl = 0u;
+    if (lane < 9u) {
+      const uint32_t sc = staged_cell(g, lane / 3u, hy + lane % 3u, hx);
+      rs = S.off[sc];
+      rl = S.off[sc + 3u] - rs;
+    }
+    const uint32_t rincl = wave_incl_scan_u32(rl, (int)lane);
+    const uint32_t rdv = rs + rl - rincl;
+    uint32_t rend[9], rdelta[9];
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+      rend[r] = readlane_u32(rincl, r);
+      rdelta[r] = readlane_u32(rdv, r);
+    }
+    const uint32_t T = rend[8];
+    uint32_t found = 0u;  // list length of home atom k0 + lane
+    for (uint32_t c0 = 0; c0 < T; c0 += 64u * kGroup) {
+      // kGroup chunks of 64 candidates, one candidate a lane, in registers
+      uint32_t q[kGroup];
+      double cx[kGroup], cy[kGroup], cz[kGroup];
+#pragma unroll
+      for (int c = 0; c < kGroup; c++) {
+        const uint32_t a = c0 + 64u * (uint32_t)c, s = a + lane;
+        q[c] = kListPad;
+#pragma unroll
+        for (int r = 8; r >= 0; r--)  // the first run that ends past s
+          if (rend[r] > a && (r == 0 || rend[r - 1] < a + 64u) && s < rend[r]) q[c] = s + rdelta[r];
+        const bool valid = s < T;
+        cx[c] = cy[c] = cz[c] = kFar;
+        if (staged) {
+          const uint32_t qs = valid ? q[c] : kListPad;
+          cx[c] = S.x[qs];
+          cy[c] = S.y[qs];
+          cz[c] = S.z[qs];
+        } else if (valid) {
+          staged_pos(R, S, false, q[c], X, Y, Z, cx[c], cy[c], cz[c]);
+        }
+      }
+      const uint32_t nc = (T - c0 + 63u) / 64u;  // chunks of this group that hold candidates
+      for (uint32_t k = k0; k < k1; k++) {
+        const uint32_t e = ebase + k;
+        double xi, yi, zi;
+        staged_pos(R, S, staged, e, X, Y, Z, xi, yi, zi);
+        uint32_t f = readlane_u32(found, k - k0);
+        uint16_t* lst = B.nbr + list_base(R, lbase + k);
+#pragma unroll
+        for (int c = 0; c < kGroup; c++) {
+          if ((uint32_t)c < nc) {
+            // a ballot marks the candidates within rc + skin (not the atom
+            // itself); each hit goes at the list length plus its rank
+            const bool hit = q[c] != e && dist2(cx[c] - xi, cy[c] - yi, cz[c] - zi) < R.list_r2;
+            const unsigned long long mask = __builtin_amdgcn_ballot_w64(hit);
+            if (mask != 0ull) {
               const uint32_t pos = f + lane_rank(mask);
-              uint16_t* lst = B.nbr + list_base(R, lbase + k);
-              if (hit && pos < cap) lst[((pos >> 3) << 9) | (pos & 7u)] = (uint16_t)q;
-              found += lane == k - t0 ? (uint32_t)__builtin_popcountll(mask) : 0u;
+              if (hit && pos < cap) lst[list_entry(pos)] = (uint16_t)q[c];
+              f += (uint32_t)__builtin_popcountll(mask);
             }
           }
         }
+        found = lane == k - k0 ? f : found;
+      }
     }
     // Past the capacity the list keeps its first `cap` entries and the
     // overflow is reported (lennard_jones.wgsl:494-497). The last octet is
     // filled up with pads.
-    if (t0 + lane < t1) {
-      const Home a = home_atom(g, S, t0 + lane);
-      const uint32_t n = found < cap ? found : cap;
-      uint16_t* lst = B.nbr + list_base(R, a.slot - R.atom_off);
-      for (uint32_t p = n; (p & 7u) != 0u; p++) lst[list_entry(p)] = (uint16_t)kListPad;
-      B.nbr_count[a.slot] = n;
+    if (lane < k1 - k0) {
+      const uint32_t li = lbase + k0 + lane;
+      const uint32_t len = found < cap ? found : cap;
+      uint16_t* lst = B.nbr + list_base(R, li);
+      for (uint32_t p = len; (p & 7u) != 0u; p++) lst[list_entry(p)] = (uint16_t)kListPad;
+      B.nbr_count[R.atom_off + li] = len;
       if (found > cap) atomicMax(&B.ctl[g.rep].overflow, found);
     }
   }
@@ -1015,7 +1076,7 @@ __device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica&
 // New accelerations and (kick) the second half kick v += a dt/2, one
 // workgroup per brick.
 template <int PREC, bool SF>
-__global__ void __launch_bounds__(kBlock) k_force(Buffers B, int kick) {
+__global__ void __launch_bounds__(kBrickThreads) k_force(Buffers B, int kick) {
   const uint32_t gb = dealt_index(blockIdx.x);
   if (gb >= B.n_bricks) return;
   const Brick g = B.bricks[gb];
@@ -1023,7 +1084,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Buffers B, int kick) {
   __shared__ Stage S;
   const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
   const uint32_t H = S.hoff[g.gx * g.gy];
-  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
     double xi, yi, zi;
@@ -1050,9 +1111,9 @@ __global__ void __launch_bounds__(kBlock) k_force(Buffers B, int kick) {
 
 // Adds slot k of every thread (k < n <= kSlots) into dst[k]: wave sums,
 // then one 64-bit atomic per slot for the workgroup.
-template <int n = kSlots>
+template <int n = kSlots, int threads = kBlock>
 __device__ __forceinline__ void block_add_slots(long long v[kSlots], long long* dst) {
-  __shared__ long long part[kBlock / 64][kSlots];
+  __shared__ long long part[threads / 64][kSlots];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < n; k++) {
@@ -1063,13 +1124,13 @@ __device__ __forceinline__ void block_add_slots(long long v[kSlots], long long* 
   if (threadIdx.x < (unsigned)n) {
     long long s = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; w++) s += part[w][threadIdx.x];
+    for (int w = 0; w < threads / 64; w++) s += part[w][threadIdx.x];
     if (s != 0) atomicAdd((unsigned long long*)(dst + threadIdx.x), (unsigned long long)s);
   }
 }
 
 template <int PREC, bool SF>
-__global__ void __launch_bounds__(kBlock) k_measure(Buffers B) {
+__global__ void __launch_bounds__(kBrickThreads) k_measure(Buffers B) {
   const uint32_t gb = dealt_index(blockIdx.x);
   if (gb >= B.n_bricks) return;
   const Brick g = B.bricks[gb];
@@ -1078,7 +1139,7 @@ __global__ void __launch_bounds__(kBlock) k_measure(Buffers B) {
   const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
   const uint32_t H = S.hoff[g.gx * g.gy];
   long long v[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
     double xi, yi, zi;
@@ -1095,7 +1156,7 @@ __global__ void __launch_bounds__(kBlock) k_measure(Buffers B) {
     v[6] += a.near >= R.near_count ? 1 : 0;
     v[7] += (long long)a.within;
   }
-  block_add_slots(v, B.slots + (size_t)g.rep * kSlots);
+  block_add_slots<kSlots, kBrickThreads>(v, B.slots + (size_t)g.rep * kSlots);
 }
 
 // ---------------------------------------------------------------------------
@@ -1132,7 +1193,7 @@ struct ProfileSink {
 };
 
 template <bool SF>
-__global__ void __launch_bounds__(kBlock) k_profile(Buffers B) {
+__global__ void __launch_bounds__(kBrickThreads) k_profile(Buffers B) {
   const uint32_t gb = dealt_index(blockIdx.x);
   if (gb >= B.n_bricks) return;
   const Brick g = B.bricks[gb];
@@ -1144,13 +1205,13 @@ __global__ void __launch_bounds__(kBlock) k_profile(Buffers B) {
   const uint32_t nsums = bins * kProfileSums + 3u;
   const ProfileSink sink{acc, B.profile + R.prof_off, bins <= kProfileLdsBins};
   if (sink.use_lds)
-    for (uint32_t k = threadIdx.x; k < nsums; k += kBlock) acc[k] = 0;
+    for (uint32_t k = threadIdx.x; k < nsums; k += kBrickThreads) acc[k] = 0;
   const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
   const double width = R.bin_width;
   const int ibins = (int)bins;
   long long whole[3] = {0, 0, 0};
   const uint32_t H = S.hoff[g.gx * g.gy];
-  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
     double xi, yi, zi;
@@ -1222,7 +1283,7 @@ __global__ void __launch_bounds__(kBlock) k_profile(Buffers B) {
   for (int a = 0; a < 3; a++) sink.add(bins * kProfileSums + (uint32_t)a, whole[a]);
   if (sink.use_lds) {
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nsums; k += kBlock)
+    for (uint32_t k = threadIdx.x; k < nsums; k += kBrickThreads)
       if (acc[k] != 0) atomicAdd((unsigned long long*)(sink.global + k), (unsigned long long)acc[k]);
   }
 }
@@ -1236,7 +1297,7 @@ __global__ void __launch_bounds__(kBlock) k_profile(Buffers B) {
 // ---------------------------------------------------------------------------
 
 template <bool SF>
-__global__ void __launch_bounds__(kBlock) k_heat_current(Buffers B, uint32_t sample) {
+__global__ void __launch_bounds__(kBrickThreads) k_heat_current(Buffers B, uint32_t sample) {
   const uint32_t gb = dealt_index(blockIdx.x);
   if (gb >= B.n_bricks) return;
   const Brick g = B.bricks[gb];
@@ -1246,7 +1307,7 @@ __global__ void __launch_bounds__(kBlock) k_heat_current(Buffers B, uint32_t sam
   const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
   long long jsum[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t H = S.hoff[g.gx * g.gy];
-  for (uint32_t h = threadIdx.x; h < H; h += kBlock) {
+  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
     double xi, yi, zi;
@@ -1281,7 +1342,7 @@ __global__ void __launch_bounds__(kBlock) k_heat_current(Buffers B, uint32_t sam
 #pragma unroll
     for (int a = 0; a < 3; a++) jsum[a] += to_fixed(__builtin_fma(e, v[a], 0.5 * pr[a]), 4294967296.0);
   }
-  block_add_slots<3>(jsum, B.hc + 3ull * sample);
+  block_add_slots<3, kBrickThreads>(jsum, B.hc + 3ull * sample);
 }
 
 __global__ void __launch_bounds__(kBlock) k_zero_slots(Buffers B) {
@@ -1360,9 +1421,9 @@ hipError_t launch_heat_exchange(const Buffers& B, hipStream_t s) {
 hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, hipStream_t s) {
   if (!B.n_bricks) return hipSuccess;
   if (shifted)
-    hipLaunchKernelGGL((k_heat_current<true>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B, sample);
+    hipLaunchKernelGGL((k_heat_current<true>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B, sample);
   else
-    hipLaunchKernelGGL((k_heat_current<false>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B, sample);
+    hipLaunchKernelGGL((k_heat_current<false>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B, sample);
   return hipGetLastError();
 }
 
@@ -1373,7 +1434,7 @@ hipError_t launch_rebuild(const Buffers& B, hipStream_t s) {
   hipLaunchKernelGGL(k_scatter, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_sort, grid1(dealt_grid(B.n_sort_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_reorder, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
-  hipLaunchKernelGGL(k_build, grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(k_build, grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
   return hipGetLastError();
 }
 
@@ -1383,14 +1444,14 @@ hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick
   const int k = kick ? 1 : 0;
   if (precision == 0) {
     if (shifted)
-      hipLaunchKernelGGL((k_force<0, true>), grid1(dealt_grid(nb)), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<0, true>), grid1(dealt_grid(nb)), dim3(kBrickThreads), 0, s, B, k);
     else
-      hipLaunchKernelGGL((k_force<0, false>), grid1(dealt_grid(nb)), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<0, false>), grid1(dealt_grid(nb)), dim3(kBrickThreads), 0, s, B, k);
   } else {
     if (shifted)
-      hipLaunchKernelGGL((k_force<1, true>), grid1(dealt_grid(nb)), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<1, true>), grid1(dealt_grid(nb)), dim3(kBrickThreads), 0, s, B, k);
     else
-      hipLaunchKernelGGL((k_force<1, false>), grid1(dealt_grid(nb)), dim3(kBlock), 0, s, B, k);
+      hipLaunchKernelGGL((k_force<1, false>), grid1(dealt_grid(nb)), dim3(kBrickThreads), 0, s, B, k);
   }
   return hipGetLastError();
 }
@@ -1401,14 +1462,14 @@ hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStre
   if (!B.n_atom_blocks) return hipGetLastError();
   if (precision == 0) {
     if (shifted)
-      hipLaunchKernelGGL((k_measure<0, true>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<0, true>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
     else
-      hipLaunchKernelGGL((k_measure<0, false>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<0, false>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
   } else {
     if (shifted)
-      hipLaunchKernelGGL((k_measure<1, true>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<1, true>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
     else
-      hipLaunchKernelGGL((k_measure<1, false>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
+      hipLaunchKernelGGL((k_measure<1, false>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
   }
   return hipGetLastError();
 }
@@ -1416,9 +1477,9 @@ hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStre
 hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s) {
   if (!B.n_bricks) return hipSuccess;
   if (shifted)
-    hipLaunchKernelGGL((k_profile<true>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL((k_profile<true>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
   else
-    hipLaunchKernelGGL((k_profile<false>), grid1(dealt_grid(B.n_bricks)), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL((k_profile<false>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
   return hipGetLastError();
 }
 

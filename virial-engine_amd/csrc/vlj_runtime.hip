@@ -341,18 +341,24 @@ int prime_if_needed(vlj_sim* s) {
 // absorbs density fluctuations. A brick that still overflows runs a slower
 // path through global memory.
 // Brick shape of each z layer: the largest gx x gy (home cells, up to
-// kBrickMax each way) whose every brick stages at most 85% of kStageMax
+// kBrickMax each way) whose every brick stages at most 90% of kStageMax
 // atoms. The count a brick is held to is the larger of what it stages in
 // the initial state at its own layer and at the layers above and below (a
 // density profile may drift by a cell, as a liquid-vapour interface does)
-// and 1.1 times the replica's mean occupancy (a lattice that is not yet
-// uniform evens out). Window sums come from a 2D prefix sum of the three
+// and the replica's mean occupancy (a lattice that is not yet uniform evens
+// out). Window sums come from a 2D prefix sum of the three
 // layers' counts over the periodically extended plane.
 void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vector<uint32_t>& bx,
                    std::vector<uint32_t>& by) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
-  const double limit = 0.85 * kStageMax;
-  const double mean = (double)R.count / (double)R.total_cells;
+#ifndef VLJ_STAGE_FILL
+#define VLJ_STAGE_FILL 0.9
+#endif
+#ifndef VLJ_MEAN_FLOOR
+#define VLJ_MEAN_FLOOR 1.0
+#endif
+  const double limit = VLJ_STAGE_FILL * kStageMax;
+  const double mean = VLJ_MEAN_FLOOR * (double)R.count / (double)R.total_cells;
   const int ex = nx + 2, ey = ny + 2;  // plane extended by one cell each side
   const size_t plane = (size_t)(ex + 1) * (ey + 1);
   std::vector<uint64_t> pre(plane * nz);
@@ -386,7 +392,7 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
       for (int gx = gy; gx <= std::min(kBrickMax, nx); gx++) {
         const uint32_t home = (uint32_t)(gx * gy);
         if (home <= best) continue;
-        double worst = 1.1 * mean * 3.0 * (gx + 2) * (gy + 2);
+        double worst = mean * 3.0 * (gx + 2) * (gy + 2);
         for (int dz = -1; dz <= 1 && worst <= limit; dz++) {
           const int zz = (z + dz + nz) % nz;
           for (int y0 = 0; y0 < ny && worst <= limit; y0 += gy)

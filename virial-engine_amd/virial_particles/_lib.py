@@ -10,7 +10,8 @@ import ctypes as C
 import os
 
 PACKAGE_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PACKAGE_ROOT, "libvirial_lj.so")
+# VLJ_LIBRARY selects another build of the same library (a tuning variant).
+LIB_PATH = os.environ.get("VLJ_LIBRARY", os.path.join(PACKAGE_ROOT, "libvirial_lj.so"))
 
 OK = 0
 E_INVALID = 1
