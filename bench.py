@@ -114,10 +114,20 @@ def run_batch(lj, specs, meta, steps, warmup, block_steps, blocks):
     b.run(warmup)
     samples = []
     stepping = 0.0
+    atoms = sum(m["n"] for m in meta)
+    segment, seg_time, seg_steps = [], 0.0, 0  # the rate over the run, in stretches of ~500 steps
     for _ in range(max(steps // block_steps, 1)):
         t0 = time.perf_counter()
         b.run(block_steps)
-        stepping += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        stepping += dt
+        seg_time += dt
+        seg_steps += block_steps
+        if seg_steps >= 500:
+            c = b.debug_counters()
+            segment.append({"steps": seg_steps, "atom_steps_per_s": atoms * seg_steps / seg_time,
+                            "cut_builds": c["stage_overflow_builds"], "rebuilds": c["rebuilds"]})
+            seg_time, seg_steps = 0.0, 0
         ms = b.measure_now()
         samples.append([m.pressure() * m.volume / (m.count * meta[r]["T"]) for r, m in enumerate(ms)])
     last = b.measure_now()
@@ -128,7 +138,6 @@ def run_batch(lj, specs, meta, steps, warmup, block_steps, blocks):
     means = z[:usable].reshape(blocks, -1, z.shape[1]).mean(axis=1)
     zmean = means.mean(axis=0)
     zse = means.std(axis=0, ddof=1) / np.sqrt(blocks)
-    atoms = sum(m["n"] for m in meta)
     done = max(steps // block_steps, 1) * block_steps
     return {
         "replicas": len(specs),
@@ -141,6 +150,7 @@ def run_batch(lj, specs, meta, steps, warmup, block_steps, blocks):
                    for r, m in enumerate(meta)],
         "list_overflow": int(max(m.list_overflow for m in last)),
         "counters": counters,
+        "segments": segment,
     }
 
 

@@ -240,9 +240,11 @@ int vlj_debug_lists(vlj_sim *sim, uint32_t *counts, uint32_t *entries);
 /* Derived integers of replica 0: cells[3], total_cells, capacity, rebuilds. */
 int vlj_debug_grid(const vlj_sim *sim, uint32_t out[6]);
 /* Totals over the handle's replicas: list rebuilds, brick list builds that
- * staged from global memory (the neighbourhood outgrew LDS), the largest
- * list overflow, and the number of bricks. No counterpart in the reference. */
-int vlj_debug_counters(const vlj_sim *sim, uint64_t out[4]);
+ * cut the brick into parts (its neighbourhood outgrew LDS), the largest list
+ * overflow, the number of bricks, and list builds of single home cells that
+ * read global memory (not even one cell's neighbourhood fits). No
+ * counterpart in the reference. */
+int vlj_debug_counters(const vlj_sim *sim, uint64_t out[5]);
 /* Derived reals of replica 0: energy_scale, momentum_scale, shift,
  * force_shift, c1, c2. */
 int vlj_debug_params(const vlj_sim *sim, double out[6]);

@@ -163,7 +163,39 @@ def test_dense_cluster_takes_the_global_memory_path(oracle, lj):
     counts = np.diff(ref.cells()[1].astype(np.int64)).reshape(10, 10, 10)
     hood = sum(np.roll(counts, (a, b, c), axis=(0, 1, 2)) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1))
     assert hood.max() > 3200
-    assert sim.debug_counters()["stage_overflow_builds"] > 0
+    assert sim.debug_counters()["global_cell_builds"] > 0
+
+
+def test_bricks_cut_into_parts_stay_exact(oracle, lj, monkeypatch):
+    # bricks planned far beyond what LDS holds (8 x 8 home cells of a
+    # liquid stage ~5700 atoms): each is cut into runs of home rows; lists,
+    # forces, measurements, profiles and the heat current stay exact
+    monkeypatch.setenv("VLJ_STAGE_FILL", "100")
+    size = [24.0, 24.0, 12.0]
+    count = int(round(0.8442 * np.prod(size)))
+    pos, vel = oracle.lattice_box(count, size, 1.0, 5)
+    st = lj.Settings(size, 2.5, 0.002, shifted_force=True, profile_bins=6)
+    sim = lj.LennardJones((pos, vel), st)
+    ref = oracle.Sim(pos, vel, size, dt=0.002, shifted_force=True)
+    ref.set_profile_bins(6)
+    assert sim.debug_grid()["cells"] == [8, 8, 4]
+    sim.record_heat_current(2)
+    ref.record_heat_current(2)
+    for steps in (1, 60):
+        sim.run(steps)
+        ref.run(steps)
+        m, r = sim.measure_now(), ref.measure()
+        assert list(m.raw) == list(r.raw)
+        np.testing.assert_array_equal(sim.debug_state()[0], ref.state()[0])
+        sim.sample_profile()
+        ref.sample_profile()
+        sim.sample_heat_current()
+        ref.sample_heat_current()
+    np.testing.assert_array_equal(sim.read_profile_raw()[0], ref.read_profile()[0])
+    np.testing.assert_array_equal(sim.read_heat_current_raw(), ref.read_heat_current())
+    c = sim.debug_counters()
+    assert c["bricks"] == 4 and c["stage_overflow_builds"] >= 4 * c["rebuilds"]
+    assert c["global_cell_builds"] == 0
 
 
 def test_two_cells_per_axis(oracle, lj):

@@ -110,7 +110,7 @@ def test_profile_global_memory_staging(oracle, lj):
     sim.sample_profile()
     ref.sample_profile()
     np.testing.assert_array_equal(sim.read_profile_raw()[0], ref.read_profile()[0])
-    assert sim.debug_counters()["stage_overflow_builds"] > 0
+    assert sim.debug_counters()["global_cell_builds"] > 0
 
 
 def test_batch_profiles_match_single_runs(oracle, lj):

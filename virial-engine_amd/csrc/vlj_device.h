@@ -33,6 +33,8 @@ constexpr int kBrickMax = 8;      // home cells of a brick along x and along y, 
 constexpr int kStageCells = 3 * (kBrickMax + 2) * (kBrickMax + 2);  // staged cells, at most
 constexpr int kHomeCells = kBrickMax * kBrickMax;                    // home cells, at most
 static_assert(kHomeCells <= 64, "one lane of a wave per home cell");
+constexpr uint32_t kPartsGrid = 128;  // workgroups of a launch over the parts of cut bricks
+constexpr uint32_t kCutGroup = VLJ_BRICK_THREADS;  // bricks per group with a count of cut bricks
 static_assert(kStageCells <= 2 * kBrickThreads, "staging takes two cells per thread");
 #ifndef VLJ_STAGE_MAX
 #define VLJ_STAGE_MAX 3200
@@ -90,7 +92,9 @@ struct Control {
   uint32_t rebuilds;
   uint32_t overflow;
   uint32_t built;
-  uint32_t stage_overflow;        // brick list builds that staged from global memory
+  uint32_t stage_overflow;        // brick list builds that cut the brick into parts
+  uint32_t stage_global;          // ... and parts of one home cell that read global memory
+  uint32_t pad_;
   double xi, eta, sm1;            // Nose-Hoover friction, its integral, velocity scale - 1
 };
 
@@ -113,6 +117,9 @@ struct Buffers {
   const Range* scan_blocks;
   const Range* sort_blocks;
   const Brick* bricks;        // list build, forces and measurements
+  uint32_t* cut;              // per brick: parts it was cut into at its replica's last list build
+  uint16_t* parts;            // per brick: kHomeCells packed parts
+  uint32_t* cut_groups;       // per group of kCutGroup bricks: bricks cut
   uint32_t n_atom_blocks, n_scan_blocks, n_sort_blocks, n_bricks, n_replicas;
   unsigned long long* bmax;   // per atom block: max moved^2 (FP64 bits)
   long long* kepart;          // per atom block: kinetic energy in fixed point (Nose-Hoover)

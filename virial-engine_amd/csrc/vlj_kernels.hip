@@ -717,12 +717,132 @@ __device__ __forceinline__ void staged_pos(const Replica& R, const Stage& S, boo
   }
 }
 
+// A value that is the same in every lane, as a scalar.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// A brick whose neighbourhood holds more than kStageMax atoms (the density
+// rose since the bricks were planned) is cut into parts: runs of whole home
+// rows, each run as long as still stages at most kStageMax atoms, and a home
+// row that does not fit alone cut the same way into runs of home cells. A
+// part of a single home cell that still does not fit reads its positions
+// from global memory. Every kernel that reads the lists cuts a brick the
+// same way, from the same cell counts, so the staged indices of a list refer
+// to the staging of its part. A part is packed as x0 | y0 << 4 | w << 8 |
+// h << 12 (home-cell offsets and sizes within the brick).
+__device__ __forceinline__ uint32_t part_rows(const Brick& g, const Stage& S, uint32_t y0, uint32_t h) {
+  const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
+  uint32_t n = 0;
+  for (uint32_t dz = 0; dz < 3u; dz++) n += S.off[(dz * sy + y0 + h + 2u) * sx] - S.off[(dz * sy + y0) * sx];
+  return n;
+}
+
+__device__ __forceinline__ uint32_t part_cells(const Brick& g, const Stage& S, uint32_t y0, uint32_t x0, uint32_t w) {
+  const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
+  uint32_t n = 0;
+  for (uint32_t dz = 0; dz < 3u; dz++)
+    for (uint32_t yy = y0; yy < y0 + 3u; yy++) {
+      const uint32_t row = (dz * sy + yy) * sx;
+      n += S.off[row + x0 + w + 2u] - S.off[row + x0];
+    }
+  return n;
+}
+
+// The parts of brick g from its staged cell offsets (one thread).
+__device__ uint32_t cut_brick(const Brick& g, const Stage& S, uint16_t* parts) {
+  const uint32_t lim = (uint32_t)kStageMax;
+  uint32_t np = 0;
+  for (uint32_t y0 = 0; y0 < g.gy;) {
+    uint32_t h = 1;
+    while (y0 + h < g.gy && part_rows(g, S, y0, h + 1u) <= lim) h++;
+    if (h > 1u || g.gx == 1u || part_rows(g, S, y0, 1u) <= lim) {
+      parts[np++] = (uint16_t)(0u | (y0 << 4) | (g.gx << 8) | (h << 12));
+    } else {
+      for (uint32_t x0 = 0; x0 < g.gx;) {
+        uint32_t w = 1;
+        while (x0 + w < g.gx && part_cells(g, S, y0, x0, w + 1u) <= lim) w++;
+        parts[np++] = (uint16_t)(x0 | (y0 << 4) | (w << 8) | (1u << 12));
+        x0 += w;
+      }
+    }
+    y0 += h;
+  }
+  return np;
+}
+
+__device__ __forceinline__ Brick part_brick(const Brick& g, uint32_t c, const Replica& R) {
+  return Brick{g.rep, g.cell0 + ((c >> 4) & 15u) * R.cells[0] + (c & 15u), (c >> 8) & 15u, c >> 12};
+}
+
+// Workgroup-wide exclusive scan of one value per thread (kBrickThreads
+// threads); *total gets the sum. Every thread calls it.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kBrickThreads / 64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_scan_u32(v, (int)lane);
+  __syncthreads();  // earlier readers of wsum are done
+  if (lane == 63u) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < (uint32_t)(kBrickThreads / 64); w++) {
+    const uint32_t x = wsum[w];
+    before += w < wave ? x : 0u;
+    all += x;
+  }
+  *total = all;
+  return before + incl - v;
+}
+
+// Calls f(gb, code) for every part (code packed as above) of every brick
+// marked cut; every thread of the workgroup calls it, and f may hold
+// barriers. The parts of all cut bricks, in table order, are numbered 0,
+// 1, ...; workgroup w takes the parts w, w + gridDim.x, ..., so the parts
+// of one brick run side by side. Every workgroup finds the cut bricks the
+// same way: the table comes in groups of kCutGroup bricks, each group with
+// a count of its cut bricks, and only groups that hold some are read.
+template <class F>
+__device__ __forceinline__ void each_cut_part(const Buffers& B, F&& f) {
+  static_assert(kCutGroup == (uint32_t)kBrickThreads, "a thread per brick of a group");
+  __shared__ uint32_t glist[kBrickThreads];  // groups with cut bricks, of this round
+  __shared__ uint32_t blist[kBrickThreads];  // cut bricks of the current group
+  __shared__ uint32_t first[kBrickThreads];  // number of each one's first part within the group
+  const uint32_t t = threadIdx.x, G = gridDim.x;
+  const uint32_t groups = (B.n_bricks + kCutGroup - 1u) / kCutGroup;
+  uint32_t base = 0;  // parts of the groups before
+  for (uint32_t g0 = 0; g0 < groups; g0 += kBrickThreads) {
+    const uint32_t grp = g0 + t;
+    const uint32_t has = grp < groups && B.cut_groups[grp] != 0u ? 1u : 0u;
+    uint32_t n_groups;
+    const uint32_t gpos = block_excl_scan(has, &n_groups);  // past every earlier use of the lists
+    if (has) glist[gpos] = grp;
+    __syncthreads();
+    for (uint32_t k = 0; k < n_groups; k++) {
+      const uint32_t gb = uniform(glist[k]) * kCutGroup + t;
+      const uint32_t np = gb < B.n_bricks ? B.cut[gb] : 0u;
+      uint32_t n_cut, n_parts;
+      const uint32_t bpos = block_excl_scan(np != 0u ? 1u : 0u, &n_cut);
+      const uint32_t upos = block_excl_scan(np, &n_parts);
+      if (np != 0u) {
+        blist[bpos] = gb;
+        first[bpos] = upos;
+      }
+      __syncthreads();
+      for (uint32_t u = (blockIdx.x + G - base % G) % G; u < n_parts; u += G) {
+        uint32_t lo = 0, hi = n_cut - 1u;  // the last cut brick whose first part is at most u
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1u) >> 1;
+          if (uniform(first[mid]) <= u) lo = mid; else hi = mid - 1u;
+        }
+        const uint32_t cb = uniform(blist[lo]);
+        f(cb, uniform(B.parts[(size_t)cb * kHomeCells + (u - uniform(first[lo]))]));
+      }
+      base += n_parts;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Verlet lists (lennard_jones.wgsl:454-499)
 // ---------------------------------------------------------------------------
-
-// A value that is the same in every lane, as a scalar.
-__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
 // Lane rank among the set bits of a wave mask (bits of lower lanes).
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
@@ -759,24 +879,13 @@ static_assert(kPiece >= 1 && kPiece <= 64, "a lane per home atom of a work item"
 // dx, then slot (R4), and a sparse cell's 27 neighbours share chunks.
 // On a reorder the workgroup also copies its home atoms' state back from
 // the second set.
-__global__ void __launch_bounds__(kBrickThreads) k_build(Buffers B) {
-  const uint32_t gb = dealt_index(blockIdx.x);
-  if (gb >= B.n_bricks) return;
-  const Brick g = B.bricks[gb];
-  const Control& ctl = B.ctl[g.rep];
-  if (!ctl.rebuild) return;
-  const bool moved = ctl.reorder != 0u;
-  const Replica R = B.rep[g.rep];
-  const double* X = moved ? B.x2 : B.x;
-  const double* Y = moved ? B.y2 : B.y;
-  const double* Z = moved ? B.z2 : B.z;
-  __shared__ Stage S;
-  __shared__ uint32_t next_item;
-  if (threadIdx.x == 0) next_item = 0u;
-  const bool staged = stage_brick(B, R, g, X, Y, Z, S);  // ends with a barrier
-  const uint32_t total = S.total;
+// The lists of the home atoms of brick (or part) g, staged in S.
+__device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+                                           bool staged, bool moved, const double* X, const double* Y,
+                                           const double* Z, uint32_t& next_item) {
   const uint32_t cap = R.capacity;
-  if (!staged && threadIdx.x == 0) atomicAdd(&B.ctl[g.rep].stage_overflow, 1u);
+  const uint32_t total = S.total;
+  if (!staged && threadIdx.x == 0) atomicAdd(&B.ctl[g.rep].stage_global, 1u);
   const uint32_t nh = g.gx * g.gy;
   const uint32_t H = S.hoff[nh];
   for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
@@ -909,6 +1018,56 @@ __global__ void __launch_bounds__(kBrickThreads) k_build(Buffers B) {
       B.nbr_count[R.atom_off + li] = len;
       if (found > cap) atomicMax(&B.ctl[g.rep].overflow, found);
     }
+  }
+}
+
+// One workgroup per brick (PARTS false), or a grid that walks the bricks
+// marked to be cut and builds them part by part (PARTS true; launched after
+// the first, on the same step). A brick counts as cut when its
+// neighbourhood does not fit in LDS and it has more than one home cell.
+template <bool PARTS>
+__global__ void __launch_bounds__(kBrickThreads) k_build(Buffers B) {
+  __shared__ Stage S;
+  __shared__ uint32_t next_item;
+  if constexpr (!PARTS) {
+    const uint32_t gb = dealt_index(blockIdx.x);
+    if (gb >= B.n_bricks) return;
+    const Brick g = B.bricks[gb];
+    const Control& ctl = B.ctl[g.rep];
+    if (!ctl.rebuild) return;
+    const bool moved = ctl.reorder != 0u;
+    const Replica R = B.rep[g.rep];
+    const double* X = moved ? B.x2 : B.x;
+    const double* Y = moved ? B.y2 : B.y;
+    const double* Z = moved ? B.z2 : B.z;
+    if (threadIdx.x == 0) next_item = 0u;
+    const bool staged = stage_brick(B, R, g, X, Y, Z, S);  // ends with a barrier
+    const bool cut = !staged && g.gx * g.gy > 1u;
+    if (threadIdx.x == 0) {  // mark the brick and record its parts for the PARTS launches
+      const uint32_t was = B.cut[gb];
+      const uint32_t np = cut ? cut_brick(g, S, B.parts + (size_t)gb * kHomeCells) : 0u;
+      B.cut[gb] = np;
+      if ((was != 0u) != (np != 0u)) atomicAdd(&B.cut_groups[gb / kCutGroup], np != 0u ? 1u : 0xffffffffu);
+      if (cut) atomicAdd(&B.ctl[g.rep].stage_overflow, 1u);
+    }
+    if (cut) return;
+    build_part(B, R, S, g, staged, moved, X, Y, Z, next_item);
+  } else {
+    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+      const Brick brick = B.bricks[gb];
+      const Control& ctl = B.ctl[brick.rep];
+      if (!ctl.rebuild) return;
+      const bool moved = ctl.reorder != 0u;
+      const Replica R = B.rep[brick.rep];
+      const double* X = moved ? B.x2 : B.x;
+      const double* Y = moved ? B.y2 : B.y;
+      const double* Z = moved ? B.z2 : B.z;
+      const Brick g = part_brick(brick, code, R);
+      __syncthreads();  // the previous part is done with S and next_item
+      if (threadIdx.x == 0) next_item = 0u;
+      const bool staged = stage_brick(B, R, g, X, Y, Z, S);
+      build_part(B, R, S, g, staged, moved, X, Y, Z, next_item);
+    });
   }
 }
 
@@ -1076,13 +1235,8 @@ __device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica&
 // New accelerations and (kick) the second half kick v += a dt/2, one
 // workgroup per brick.
 template <int PREC, bool SF>
-__global__ void __launch_bounds__(kBrickThreads) k_force(Buffers B, int kick) {
-  const uint32_t gb = dealt_index(blockIdx.x);
-  if (gb >= B.n_bricks) return;
-  const Brick g = B.bricks[gb];
-  const Replica R = B.rep[g.rep];
-  __shared__ Stage S;
-  const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+__device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+                                           bool staged, int kick) {
   const uint32_t H = S.hoff[g.gx * g.gy];
   for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
     const Home at = home_atom(g, S, h);
@@ -1099,6 +1253,31 @@ __global__ void __launch_bounds__(kBrickThreads) k_force(Buffers B, int kick) {
       B.vy[i] = __builtin_fma(ay, R.half_dt, B.vy[i]);
       B.vz[i] = __builtin_fma(az, R.half_dt, B.vz[i]);
     }
+  }
+}
+
+// One workgroup per brick, leaving bricks that are cut (see k_build) to the
+// PARTS launch that follows.
+template <int PREC, bool SF, bool PARTS>
+__global__ void __launch_bounds__(kBrickThreads) k_force(Buffers B, int kick) {
+  __shared__ Stage S;
+  if constexpr (!PARTS) {
+    const uint32_t gb = dealt_index(blockIdx.x);
+    if (gb >= B.n_bricks) return;
+    const Brick g = B.bricks[gb];
+    const Replica R = B.rep[g.rep];
+    const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+    if (!staged && g.gx * g.gy > 1u) return;
+    force_part<PREC, SF>(B, R, S, g, staged, kick);
+  } else {
+    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+      const Brick brick = B.bricks[gb];
+      const Replica R = B.rep[brick.rep];
+      const Brick g = part_brick(brick, code, R);
+      __syncthreads();  // the previous part is done with S
+      const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+      force_part<PREC, SF>(B, R, S, g, staged, kick);
+    });
   }
 }
 
@@ -1130,33 +1309,57 @@ __device__ __forceinline__ void block_add_slots(long long v[kSlots], long long* 
 }
 
 template <int PREC, bool SF>
-__global__ void __launch_bounds__(kBrickThreads) k_measure(Buffers B) {
-  const uint32_t gb = dealt_index(blockIdx.x);
-  if (gb >= B.n_bricks) return;
-  const Brick g = B.bricks[gb];
-  const Replica R = B.rep[g.rep];
-  __shared__ Stage S;
-  const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+// Each pass over up to kBrickThreads home atoms ends in the workgroup's sum
+// (a pass's sums are not kept across the pair loop).
+__device__ __forceinline__ void measure_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+                                             bool staged, long long* dst) {
   const uint32_t H = S.hoff[g.gx * g.gy];
-  long long v[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
-    const Home at = home_atom(g, S, h);
-    const uint32_t i = at.slot;
-    double xi, yi, zi;
-    staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
-    const PairAcc a = interactions<PREC, SF, true>(B, R, S, staged, i, xi, yi, zi);
-    const double vx = B.vx[i], vy = B.vy[i], vz = B.vz[i];
-    const double ke = 0.5 * dist2(vx, vy, vz);
-    v[0] += to_fixed(ke, R.energy_fixed);
-    v[1] += to_fixed(a.pe, R.energy_fixed);
-    v[2] += to_fixed(a.w, R.energy_fixed);
-    v[3] += to_fixed(vx, R.momentum_fixed);
-    v[4] += to_fixed(vy, R.momentum_fixed);
-    v[5] += to_fixed(vz, R.momentum_fixed);
-    v[6] += a.near >= R.near_count ? 1 : 0;
-    v[7] += (long long)a.within;
+  for (uint32_t h0 = 0; h0 < H; h0 += kBrickThreads) {
+    long long v[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t h = h0 + threadIdx.x;
+    if (h < H) {
+      const Home at = home_atom(g, S, h);
+      const uint32_t i = at.slot;
+      double xi, yi, zi;
+      staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
+      const PairAcc a = interactions<PREC, SF, true>(B, R, S, staged, i, xi, yi, zi);
+      const double vx = B.vx[i], vy = B.vy[i], vz = B.vz[i];
+      const double ke = 0.5 * dist2(vx, vy, vz);
+      v[0] = to_fixed(ke, R.energy_fixed);
+      v[1] = to_fixed(a.pe, R.energy_fixed);
+      v[2] = to_fixed(a.w, R.energy_fixed);
+      v[3] = to_fixed(vx, R.momentum_fixed);
+      v[4] = to_fixed(vy, R.momentum_fixed);
+      v[5] = to_fixed(vz, R.momentum_fixed);
+      v[6] = a.near >= R.near_count ? 1 : 0;
+      v[7] = (long long)a.within;
+    }
+    block_add_slots<kSlots, kBrickThreads>(v, dst);
+    __syncthreads();  // the sum's LDS is free for the next pass
   }
-  block_add_slots<kSlots, kBrickThreads>(v, B.slots + (size_t)g.rep * kSlots);
+}
+
+template <int PREC, bool SF, bool PARTS>
+__global__ void __launch_bounds__(kBrickThreads) k_measure(Buffers B) {
+  __shared__ Stage S;
+  if constexpr (!PARTS) {
+    const uint32_t gb = dealt_index(blockIdx.x);
+    if (gb >= B.n_bricks) return;
+    const Brick g = B.bricks[gb];
+    const Replica R = B.rep[g.rep];
+    const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+    if (!staged && g.gx * g.gy > 1u) return;
+    measure_part<PREC, SF>(B, R, S, g, staged, B.slots + (size_t)g.rep * kSlots);
+  } else {
+    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+      const Brick brick = B.bricks[gb];
+      const Replica R = B.rep[brick.rep];
+      const Brick g = part_brick(brick, code, R);
+      __syncthreads();  // the previous part is done with S
+      const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+      measure_part<PREC, SF>(B, R, S, g, staged, B.slots + (size_t)g.rep * kSlots);
+    });
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1193,23 +1396,11 @@ struct ProfileSink {
 };
 
 template <bool SF>
-__global__ void __launch_bounds__(kBrickThreads) k_profile(Buffers B) {
-  const uint32_t gb = dealt_index(blockIdx.x);
-  if (gb >= B.n_bricks) return;
-  const Brick g = B.bricks[gb];
-  const Replica R = B.rep[g.rep];
+__device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+                                             bool staged, const ProfileSink& sink, long long whole[3]) {
   const uint32_t bins = R.prof_bins;
-  if (bins == 0) return;  // the whole workgroup leaves together
-  __shared__ Stage S;
-  __shared__ long long acc[kProfileLdsBins * kProfileSums + 3];
-  const uint32_t nsums = bins * kProfileSums + 3u;
-  const ProfileSink sink{acc, B.profile + R.prof_off, bins <= kProfileLdsBins};
-  if (sink.use_lds)
-    for (uint32_t k = threadIdx.x; k < nsums; k += kBrickThreads) acc[k] = 0;
-  const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
   const double width = R.bin_width;
   const int ibins = (int)bins;
-  long long whole[3] = {0, 0, 0};
   const uint32_t H = S.hoff[g.gx * g.gy];
   for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
     const Home at = home_atom(g, S, h);
@@ -1279,12 +1470,47 @@ __global__ void __launch_bounds__(kBrickThreads) k_profile(Buffers B) {
 #pragma unroll
     for (int a = 0; a < 3; a++) whole[a] += to_fixed(total[a], kProfileOne);
   }
+}
+
+// Adds the workgroup's sums of one brick to the replica's profile.
+__device__ __forceinline__ void profile_flush(const ProfileSink& sink, uint32_t bins, const long long whole[3]) {
+  const uint32_t nsums = bins * kProfileSums + 3u;
 #pragma unroll
   for (int a = 0; a < 3; a++) sink.add(bins * kProfileSums + (uint32_t)a, whole[a]);
   if (sink.use_lds) {
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < nsums; k += kBrickThreads)
-      if (acc[k] != 0) atomicAdd((unsigned long long*)(sink.global + k), (unsigned long long)acc[k]);
+      if (sink.lds[k] != 0) atomicAdd((unsigned long long*)(sink.global + k), (unsigned long long)sink.lds[k]);
+  }
+}
+
+template <bool SF, bool PARTS>
+__global__ void __launch_bounds__(kBrickThreads) k_profile(Buffers B) {
+  __shared__ Stage S;
+  __shared__ long long acc[kProfileLdsBins * kProfileSums + 3];
+  // brick gb whole (code 0) or one part of it
+  auto profile = [&](uint32_t gb, uint32_t code) {
+    const Brick brick = B.bricks[gb];
+    const Replica R = B.rep[brick.rep];
+    const uint32_t bins = R.prof_bins;
+    if (bins == 0) return;  // the whole workgroup together
+    const uint32_t nsums = bins * kProfileSums + 3u;
+    const ProfileSink sink{acc, B.profile + R.prof_off, bins <= kProfileLdsBins};
+    const Brick g = PARTS ? part_brick(brick, code, R) : brick;
+    if (PARTS) __syncthreads();  // the previous part is done with S and acc
+    if (sink.use_lds)
+      for (uint32_t k = threadIdx.x; k < nsums; k += kBrickThreads) acc[k] = 0;
+    const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
+    if (!PARTS && !staged && g.gx * g.gy > 1u) return;
+    long long whole[3] = {0, 0, 0};
+    profile_part<SF>(B, R, S, g, staged, sink, whole);
+    profile_flush(sink, bins, whole);
+  };
+  if constexpr (!PARTS) {
+    const uint32_t gb = dealt_index(blockIdx.x);
+    if (gb < B.n_bricks) profile(gb, 0u);
+  } else {
+    each_cut_part(B, profile);
   }
 }
 
@@ -1297,15 +1523,8 @@ __global__ void __launch_bounds__(kBrickThreads) k_profile(Buffers B) {
 // ---------------------------------------------------------------------------
 
 template <bool SF>
-__global__ void __launch_bounds__(kBrickThreads) k_heat_current(Buffers B, uint32_t sample) {
-  const uint32_t gb = dealt_index(blockIdx.x);
-  if (gb >= B.n_bricks) return;
-  const Brick g = B.bricks[gb];
-  const Replica R = B.rep[g.rep];
-  if (R.hc_cap == 0u || sample >= R.hc_cap) return;
-  __shared__ Stage S;
-  const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-  long long jsum[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
+__device__ __forceinline__ void heat_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+                                          bool staged, long long jsum[kSlots]) {
   const uint32_t H = S.hoff[g.gx * g.gy];
   for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
     const Home at = home_atom(g, S, h);
@@ -1342,7 +1561,35 @@ __global__ void __launch_bounds__(kBrickThreads) k_heat_current(Buffers B, uint3
 #pragma unroll
     for (int a = 0; a < 3; a++) jsum[a] += to_fixed(__builtin_fma(e, v[a], 0.5 * pr[a]), 4294967296.0);
   }
-  block_add_slots<3, kBrickThreads>(jsum, B.hc + 3ull * sample);
+}
+
+template <bool SF, bool PARTS>
+__global__ void __launch_bounds__(kBrickThreads) k_heat_current(Buffers B, uint32_t sample) {
+  __shared__ Stage S;
+  if constexpr (!PARTS) {
+    const uint32_t gb = dealt_index(blockIdx.x);
+    if (gb >= B.n_bricks) return;
+    const Brick g = B.bricks[gb];
+    const Replica R = B.rep[g.rep];
+    if (R.hc_cap == 0u || sample >= R.hc_cap) return;
+    const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+    if (!staged && g.gx * g.gy > 1u) return;
+    long long jsum[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
+    heat_part<SF>(B, R, S, g, staged, jsum);
+    block_add_slots<3, kBrickThreads>(jsum, B.hc + 3ull * sample);
+  } else {
+    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+      const Brick brick = B.bricks[gb];
+      const Replica R = B.rep[brick.rep];
+      if (R.hc_cap == 0u || sample >= R.hc_cap) return;
+      const Brick g = part_brick(brick, code, R);
+      __syncthreads();  // the previous part is done with S and the sums
+      const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+      long long jsum[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
+      heat_part<SF>(B, R, S, g, staged, jsum);
+      block_add_slots<3, kBrickThreads>(jsum, B.hc + 3ull * sample);
+    });
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_zero_slots(Buffers B) {
@@ -1418,12 +1665,19 @@ hipError_t launch_heat_exchange(const Buffers& B, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Grid of a launch over the bricks that are cut into parts.
+inline dim3 parts_grid(const Buffers& B) { return grid1(std::min<uint32_t>(B.n_bricks, kPartsGrid)); }
+
 hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, hipStream_t s) {
   if (!B.n_bricks) return hipSuccess;
-  if (shifted)
-    hipLaunchKernelGGL((k_heat_current<true>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B, sample);
-  else
-    hipLaunchKernelGGL((k_heat_current<false>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B, sample);
+  const dim3 g = grid1(dealt_grid(B.n_bricks)), t = dim3(kBrickThreads);
+  if (shifted) {
+    hipLaunchKernelGGL((k_heat_current<true, false>), g, t, 0, s, B, sample);
+    hipLaunchKernelGGL((k_heat_current<true, true>), parts_grid(B), t, 0, s, B, sample);
+  } else {
+    hipLaunchKernelGGL((k_heat_current<false, false>), g, t, 0, s, B, sample);
+    hipLaunchKernelGGL((k_heat_current<false, true>), parts_grid(B), t, 0, s, B, sample);
+  }
   return hipGetLastError();
 }
 
@@ -1434,24 +1688,33 @@ hipError_t launch_rebuild(const Buffers& B, hipStream_t s) {
   hipLaunchKernelGGL(k_scatter, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_sort, grid1(dealt_grid(B.n_sort_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_reorder, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
-  hipLaunchKernelGGL(k_build, grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
+  if (B.n_bricks) {
+    hipLaunchKernelGGL(k_build<false>, grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
+    hipLaunchKernelGGL(k_build<true>, parts_grid(B), dim3(kBrickThreads), 0, s, B);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s) {
   if (!B.n_atom_blocks) return hipSuccess;
-  const uint32_t nb = B.n_bricks;
+  const dim3 g = grid1(dealt_grid(B.n_bricks)), t = dim3(kBrickThreads), gp = parts_grid(B);
   const int k = kick ? 1 : 0;
   if (precision == 0) {
-    if (shifted)
-      hipLaunchKernelGGL((k_force<0, true>), grid1(dealt_grid(nb)), dim3(kBrickThreads), 0, s, B, k);
-    else
-      hipLaunchKernelGGL((k_force<0, false>), grid1(dealt_grid(nb)), dim3(kBrickThreads), 0, s, B, k);
+    if (shifted) {
+      hipLaunchKernelGGL((k_force<0, true, false>), g, t, 0, s, B, k);
+      hipLaunchKernelGGL((k_force<0, true, true>), gp, t, 0, s, B, k);
+    } else {
+      hipLaunchKernelGGL((k_force<0, false, false>), g, t, 0, s, B, k);
+      hipLaunchKernelGGL((k_force<0, false, true>), gp, t, 0, s, B, k);
+    }
   } else {
-    if (shifted)
-      hipLaunchKernelGGL((k_force<1, true>), grid1(dealt_grid(nb)), dim3(kBrickThreads), 0, s, B, k);
-    else
-      hipLaunchKernelGGL((k_force<1, false>), grid1(dealt_grid(nb)), dim3(kBrickThreads), 0, s, B, k);
+    if (shifted) {
+      hipLaunchKernelGGL((k_force<1, true, false>), g, t, 0, s, B, k);
+      hipLaunchKernelGGL((k_force<1, true, true>), gp, t, 0, s, B, k);
+    } else {
+      hipLaunchKernelGGL((k_force<1, false, false>), g, t, 0, s, B, k);
+      hipLaunchKernelGGL((k_force<1, false, true>), gp, t, 0, s, B, k);
+    }
   }
   return hipGetLastError();
 }
@@ -1460,26 +1723,37 @@ hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStre
   const uint32_t zb = (B.n_replicas * kSlots + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_zero_slots, grid1(zb), dim3(kBlock), 0, s, B);
   if (!B.n_atom_blocks) return hipGetLastError();
+  const dim3 g = grid1(dealt_grid(B.n_bricks)), t = dim3(kBrickThreads), gp = parts_grid(B);
   if (precision == 0) {
-    if (shifted)
-      hipLaunchKernelGGL((k_measure<0, true>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
-    else
-      hipLaunchKernelGGL((k_measure<0, false>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
+    if (shifted) {
+      hipLaunchKernelGGL((k_measure<0, true, false>), g, t, 0, s, B);
+      hipLaunchKernelGGL((k_measure<0, true, true>), gp, t, 0, s, B);
+    } else {
+      hipLaunchKernelGGL((k_measure<0, false, false>), g, t, 0, s, B);
+      hipLaunchKernelGGL((k_measure<0, false, true>), gp, t, 0, s, B);
+    }
   } else {
-    if (shifted)
-      hipLaunchKernelGGL((k_measure<1, true>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
-    else
-      hipLaunchKernelGGL((k_measure<1, false>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
+    if (shifted) {
+      hipLaunchKernelGGL((k_measure<1, true, false>), g, t, 0, s, B);
+      hipLaunchKernelGGL((k_measure<1, true, true>), gp, t, 0, s, B);
+    } else {
+      hipLaunchKernelGGL((k_measure<1, false, false>), g, t, 0, s, B);
+      hipLaunchKernelGGL((k_measure<1, false, true>), gp, t, 0, s, B);
+    }
   }
   return hipGetLastError();
 }
 
 hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s) {
   if (!B.n_bricks) return hipSuccess;
-  if (shifted)
-    hipLaunchKernelGGL((k_profile<true>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
-  else
-    hipLaunchKernelGGL((k_profile<false>), grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
+  const dim3 g = grid1(dealt_grid(B.n_bricks)), t = dim3(kBrickThreads);
+  if (shifted) {
+    hipLaunchKernelGGL((k_profile<true, false>), g, t, 0, s, B);
+    hipLaunchKernelGGL((k_profile<true, true>), parts_grid(B), t, 0, s, B);
+  } else {
+    hipLaunchKernelGGL((k_profile<false, false>), g, t, 0, s, B);
+    hipLaunchKernelGGL((k_profile<false, true>), parts_grid(B), t, 0, s, B);
+  }
   return hipGetLastError();
 }
 

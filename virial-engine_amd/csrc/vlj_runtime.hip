@@ -335,11 +335,6 @@ int prime_if_needed(vlj_sim* s) {
   return VLJ_OK;
 }
 
-// Brick shape of a replica: the most home cells gx * gy (gx, gy <= kBrickMax
-// and within the grid) whose densest staged neighbourhood, counted on the
-// initial cell occupancy, holds at most 85% of kStageMax atoms; the margin
-// absorbs density fluctuations. A brick that still overflows runs a slower
-// path through global memory.
 // Brick shape of each z layer: the largest gx x gy (home cells, up to
 // kBrickMax each way) whose every brick stages at most 90% of kStageMax
 // atoms. The count a brick is held to is the larger of what it stages in
@@ -347,7 +342,10 @@ int prime_if_needed(vlj_sim* s) {
 // density profile may drift by a cell, as a liquid-vapour interface does)
 // and the replica's mean occupancy (a lattice that is not yet uniform evens
 // out). Window sums come from a 2D prefix sum of the three
-// layers' counts over the periodically extended plane.
+// layers' counts over the periodically extended plane. A brick that
+// outgrows LDS later on is cut into parts on the device. The environment
+// variable VLJ_STAGE_FILL, when set, replaces the 90% (a value above 1
+// plans bricks that will be cut: for tests and tuning).
 void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vector<uint32_t>& bx,
                    std::vector<uint32_t>& by) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
@@ -357,7 +355,9 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
 #ifndef VLJ_MEAN_FLOOR
 #define VLJ_MEAN_FLOOR 1.0
 #endif
-  const double limit = VLJ_STAGE_FILL * kStageMax;
+  double fill = VLJ_STAGE_FILL;
+  if (const char* f = std::getenv("VLJ_STAGE_FILL")) fill = std::atof(f);
+  const double limit = fill * kStageMax;
   const double mean = VLJ_MEAN_FLOOR * (double)R.count / (double)R.total_cells;
   const int ex = nx + 2, ey = ny + 2;  // plane extended by one cell each side
   const size_t plane = (size_t)(ex + 1) * (ey + 1);
@@ -517,6 +517,9 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   A(&sb, s->scan_blocks.size());
   A(&ob, s->sort_blocks.size());
   A(&gb, s->bricks.size());
+  A(&B.cut, s->bricks.size());
+  A(&B.parts, s->bricks.size() * kHomeCells);
+  A(&B.cut_groups, (s->bricks.size() + kCutGroup - 1) / kCutGroup);
   A(&B.bmax, s->atom_blocks.size());
   A(&B.kepart, s->atom_blocks.size());
   A(&B.hexpart, s->atom_blocks.size() * kHexSums);
@@ -580,6 +583,8 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   if (e == hipSuccess) e = hipMemcpy(ob, s->sort_blocks.data(), s->sort_blocks.size() * sizeof(Range), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(gb, s->bricks.data(), s->bricks.size() * sizeof(Brick), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(B.bmax, 0, std::max<size_t>(s->atom_blocks.size(), 1) * 8);
+  if (e == hipSuccess) e = hipMemset(B.cut, 0, std::max<size_t>(s->bricks.size(), 1) * 4);
+  if (e == hipSuccess) e = hipMemset(B.cut_groups, 0, std::max<size_t>((s->bricks.size() + kCutGroup - 1) / kCutGroup, 1) * 4);
   // Zeroed once here; afterwards the sort kernel leaves them zero.
   if (e == hipSuccess) e = hipMemset(B.cell_count, 0, std::max<size_t>(C, 1) * 4);
   if (e == hipSuccess) e = hipMemset(B.cell_cursor, 0, std::max<size_t>(C, 1) * 4);
@@ -1187,16 +1192,17 @@ int vlj_debug_grid(const vlj_sim* sim, uint32_t out[6]) {
   return VLJ_OK;
 }
 
-int vlj_debug_counters(const vlj_sim* sim, uint64_t out[4]) {
+int vlj_debug_counters(const vlj_sim* sim, uint64_t out[5]) {
   if (!sim || !out) return VLJ_E_INVALID;
   std::vector<Control> c(sim->rep.size());
   if (!c.empty() && hipMemcpy(c.data(), sim->B.ctl, c.size() * sizeof(Control), hipMemcpyDeviceToHost) != hipSuccess)
     return VLJ_E_HIP;
-  out[0] = out[1] = out[2] = 0;
+  out[0] = out[1] = out[2] = out[4] = 0;
   for (const Control& x : c) {
     out[0] += x.rebuilds;
     out[1] += x.stage_overflow;
     out[2] = std::max<uint64_t>(out[2], x.overflow);
+    out[4] += x.stage_global;
   }
   out[3] = sim->bricks.size();
   return VLJ_OK;
