@@ -172,8 +172,8 @@ int vlj_heat_flux(const vlj_sim *sim, double *flux, int32_t *has);
  * J = sum_i e_i v_i + 1/2 sum_ij (F_ij . v_i) r_ij. record makes room for
  * `capacity` samples (dropping any); sample adds one (VLJ_E_STATE "record_heat_current
  * first", "heat current series full"); read writes 3 values (x, y, z) per
- * sample, capacity counted in samples; read_raw the fixed-point sums
- * (value * 2^32). */
+ * sample, capacity counted in samples (out NULL: only the number of
+ * samples); read_raw the fixed-point sums (value * 2^32). */
 int vlj_record_heat_current(vlj_sim *sim, uint32_t capacity);
 int vlj_sample_heat_current(vlj_sim *sim);
 int vlj_read_heat_current(vlj_sim *sim, double *out, size_t capacity, uint32_t *samples);
@@ -192,6 +192,26 @@ int vlj_batch_create(const vlj_particle *all, const uint64_t *offsets,
 int vlj_batch_run(vlj_batch *batch, uint64_t steps);
 /* One measurement per replica, in replica order. */
 int vlj_batch_measure_now(vlj_batch *batch, vlj_measurement *per_replica, size_t replicas);
+
+/* ---- Checkpoint and exact resume (no counterpart in the reference, whose
+ * runs start again from particles; the on-disk formats of its app crate,
+ * crates/virial/src/data.rs, hold measurements and snapshots only). */
+/* Writes the complete state of a handle (single or batch) after the steps
+ * enqueued so far: positions, velocities, accelerations, positions at the
+ * last list build, original indices, cell tables, lists and bricks, the
+ * control words (step counter of the noise, rebuild count, Nose-Hoover xi
+ * and eta), the parameters as last set, profile sums and the heat-current
+ * series. Measurements in flight are not kept. */
+int vlj_save(vlj_sim *sim, const char *path);
+/* A new handle from a file written by vlj_save; stepping it gives the same
+ * bits as stepping the handle that saved it. VLJ_E_INVALID for a file that
+ * is not a state file, is truncated, or was written by another version. */
+int vlj_load(const char *path, vlj_sim **out);
+/* The number of replicas of a handle and, for replica `replica`, the
+ * settings it runs with (thermostat temperature as last set) and its
+ * particle count. Any output may be NULL. */
+int vlj_replica_info(const vlj_sim *sim, size_t replica, vlj_settings *settings, uint64_t *count,
+                     size_t *replicas);
 size_t vlj_batch_replicas(const vlj_batch *batch);
 /* Particles of every replica, concatenated as given to vlj_batch_create. */
 int vlj_batch_read_particles(vlj_batch *batch, vlj_particle *out, size_t count);

@@ -150,3 +150,17 @@ def test_measurement_helpers(lj):
     assert m.temperature(6.0) == 1.0
     assert m.pressure() == (6.0 + 2.0) / 24.0
     assert m.extended_energy() == -1.75
+
+
+def test_load_rejects_files_that_are_not_states(lj, tmp_path):
+    # vlj_load reads and checks the file before it touches a device
+    with pytest.raises(lj.LennardJonesError, match="could not open"):
+        lj.LennardJones.load(tmp_path / "missing.state")
+    junk = tmp_path / "junk.state"
+    junk.write_bytes(b"VLJSTATX" + bytes(200))
+    with pytest.raises(lj.LennardJonesError, match="not a state file"):
+        lj.LennardJones.load(junk)
+    short = tmp_path / "short.state"
+    short.write_bytes(b"VLJSTATE" + bytes(10))
+    with pytest.raises(lj.LennardJonesError, match="not a state file|truncated"):
+        lj.Batch.load(short)

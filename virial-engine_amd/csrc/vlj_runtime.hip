@@ -409,8 +409,10 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
 }
 
 // Builds a handle over `replicas` systems.
+// `saved` (from a state file) fixes what derive() takes from the initial
+// particles -- list capacity and fixed-point scales -- to the saved values.
 int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings* settings,
-           size_t replicas, vlj_sim** out) {
+           size_t replicas, vlj_sim** out, const Replica* saved = nullptr) {
   if (!out) return fail(nullptr, VLJ_E_INVALID, "out must not be NULL");
   *out = nullptr;
   if (replicas == 0 || !settings || !offsets)
@@ -444,6 +446,13 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     if (rc != VLJ_OK) {
       delete s;
       return fail(nullptr, rc, replicas > 1 ? "replica " + std::to_string(r) + ": " + err : err);
+    }
+    if (saved) {
+      R.capacity = saved[r].capacity;
+      R.energy_fixed = saved[r].energy_fixed;
+      R.momentum_fixed = saved[r].momentum_fixed;
+      s->host[r].energy_scale = R.energy_fixed / kTwo32;
+      s->host[r].momentum_scale = R.momentum_fixed / kTwo32;
     }
     R.index = (uint32_t)r;
     R.atom_off = atom_off;
@@ -1022,8 +1031,8 @@ static int read_series(vlj_sim* sim, std::vector<long long>& raw, uint32_t* samp
 int vlj_read_heat_current(vlj_sim* sim, double* out, size_t capacity, uint32_t* samples) {
   std::vector<long long> raw;
   int rc = read_series(sim, raw, samples);
-  if (rc != VLJ_OK) return rc;
-  if (raw.size() > 3 * capacity || (!out && !raw.empty()))
+  if (rc != VLJ_OK || !out) return rc;  // out NULL: the number of samples only
+  if (raw.size() > 3 * capacity)
     return fail(sim, VLJ_E_INVALID, "the output holds fewer samples than recorded");
   for (size_t k = 0; k < raw.size(); k++) out[k] = (double)raw[k] / kTwo32;
   return VLJ_OK;
@@ -1032,8 +1041,8 @@ int vlj_read_heat_current(vlj_sim* sim, double* out, size_t capacity, uint32_t* 
 int vlj_read_heat_current_raw(vlj_sim* sim, int64_t* out, size_t capacity, uint32_t* samples) {
   std::vector<long long> raw;
   int rc = read_series(sim, raw, samples);
-  if (rc != VLJ_OK) return rc;
-  if (raw.size() > 3 * capacity || (!out && !raw.empty()))
+  if (rc != VLJ_OK || !out) return rc;  // out NULL: the number of samples only
+  if (raw.size() > 3 * capacity)
     return fail(sim, VLJ_E_INVALID, "the output holds fewer samples than recorded");
   if (!raw.empty()) std::memcpy(out, raw.data(), raw.size() * sizeof(long long));
   return VLJ_OK;
@@ -1495,6 +1504,302 @@ int vlj_random_insertion(size_t count, const double box[3], double min_distance,
   for (size_t i = 0; i < count; i++)
     for (int k = 0; k < 3; k++) out[i].vel[k] = lattice_gaussian(&s) * st;
   if (count) zero_momentum(count, out);
+  return VLJ_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Checkpoint and exact resume. A state file holds everything that carries
+// from one step to the next, as the device holds it (slot order), so a
+// handle loaded from it steps on to the same bits as the handle that saved
+// it: positions, velocities and accelerations; positions at the last list
+// build; original indices; cell tables, lists and the bricks (with the parts
+// of cut bricks) they refer to; the control words (the step counter that
+// keys the Langevin noise, the rebuild count, Nose-Hoover xi and eta); the
+// replicas' parameters as last set (target temperature, heat exchange);
+// profile sums and the heat-current series. Measurements in flight are not
+// kept. Layout: a StateHeader, then the sections in the order written below,
+// all little-endian, each array as raw elements.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr char kStateMagic[8] = {'V', 'L', 'J', 'S', 'T', 'A', 'T', 'E'};
+constexpr uint32_t kStateVersion = 1;
+
+struct StateHeader {
+  char magic[8];
+  uint32_t version, replicas;
+  uint64_t n_total, cells_total, list_total, prof_total, n_bricks, steps;
+  uint32_t primed, hc_samples, hc_cap;
+  uint32_t sizeof_replica, sizeof_control, sizeof_settings, sizeof_brick, pad;
+};
+
+struct FileOut {
+  FILE* f = nullptr;
+  bool ok = true;
+  void put(const void* p, size_t n) {
+    if (ok && n) ok = std::fwrite(p, 1, n, f) == n;
+  }
+};
+
+struct FileIn {
+  FILE* f = nullptr;
+  bool ok = true;
+  void get(void* p, size_t n) {
+    if (ok && n) ok = std::fread(p, 1, n, f) == n;
+  }
+};
+
+template <typename T>
+int save_array(vlj_sim* s, FileOut& w, const T* dev, size_t n) {
+  std::vector<T> h(n);
+  if (n) HIP_TRY(s, hipMemcpy(h.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost));
+  w.put(h.data(), n * sizeof(T));
+  return VLJ_OK;
+}
+
+template <typename T>
+bool read_vec(FileIn& r, std::vector<T>& v, size_t n) {
+  v.resize(n);
+  r.get(v.data(), n * sizeof(T));
+  return r.ok;
+}
+
+template <typename T>
+int upload(vlj_sim* s, T* dev, const std::vector<T>& h) {
+  if (!h.empty()) HIP_TRY(s, hipMemcpy(dev, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return VLJ_OK;
+}
+
+// The state arrays of a file, in file order.
+struct StateArrays {
+  std::vector<double> f64[12];  // x y z vx vy vz ax ay az xb yb zb
+  std::vector<uint32_t> orig, cell_of, nbr_count, cell_start, cell_n, cut, cut_groups;
+  std::vector<uint16_t> nbr, parts;
+  std::vector<long long> profile, hc;
+};
+
+double* const* state_f64(Buffers& B) {
+  static thread_local double* p[12];
+  double* q[12] = {B.x, B.y, B.z, B.vx, B.vy, B.vz, B.ax, B.ay, B.az, B.xb, B.yb, B.zb};
+  for (int k = 0; k < 12; k++) p[k] = q[k];
+  return p;
+}
+
+void free_device(vlj_sim* s, void* p) {
+  if (!p) return;
+  auto it = std::find(s->allocations.begin(), s->allocations.end(), p);
+  if (it != s->allocations.end()) s->allocations.erase(it);
+  (void)hipFree(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vlj_save(vlj_sim* sim, const char* path) {
+  if (!sim || !path) return fail(sim, VLJ_E_INVALID, "sim and path must not be NULL");
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  FileOut w;
+  w.f = std::fopen(path, "wb");
+  if (!w.f) return fail(sim, VLJ_E_INVALID, std::string("could not open ") + path + " for writing");
+  const size_t R = sim->rep.size(), N = sim->n_total, C = sim->cells_total, nb = sim->bricks.size();
+  const size_t ng = (nb + kCutGroup - 1) / kCutGroup;
+  StateHeader h{};
+  std::memcpy(h.magic, kStateMagic, 8);
+  h.version = kStateVersion;
+  h.replicas = (uint32_t)R;
+  h.n_total = N;
+  h.cells_total = C;
+  h.list_total = sim->list_total;
+  h.prof_total = sim->prof_total;
+  h.n_bricks = nb;
+  h.steps = sim->steps;
+  h.primed = sim->primed ? 1u : 0u;
+  h.hc_samples = sim->hc_samples;
+  h.hc_cap = sim->rep[0].hc_cap;
+  h.sizeof_replica = sizeof(Replica);
+  h.sizeof_control = sizeof(Control);
+  h.sizeof_settings = sizeof(vlj_settings);
+  h.sizeof_brick = sizeof(Brick);
+  w.put(&h, sizeof h);
+  for (const HostReplica& x : sim->host) w.put(&x.settings, sizeof(vlj_settings));
+  w.put(sim->rep.data(), R * sizeof(Replica));
+  for (const HostReplica& x : sim->host) {
+    w.put(&x.prof_samples, sizeof(uint64_t));
+    w.put(x.brick_x.data(), x.brick_x.size() * sizeof(uint32_t));
+    w.put(x.brick_y.data(), x.brick_y.size() * sizeof(uint32_t));
+  }
+  w.put(sim->bricks.data(), nb * sizeof(Brick));
+  int rc = save_array(sim, w, sim->B.ctl, R);
+  double* const* f = state_f64(sim->B);
+  for (int k = 0; k < 12 && rc == VLJ_OK; k++) rc = save_array(sim, w, f[k], N);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.orig, N);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.cell_of, N);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.nbr_count, N);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.cell_start, C);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.cell_n, C);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.nbr, sim->list_total);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.cut, nb);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.parts, nb * kHomeCells);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.cut_groups, ng);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.profile, sim->prof_total);
+  if (rc == VLJ_OK) rc = save_array(sim, w, sim->B.hc, (size_t)h.hc_cap * 3);
+  const bool closed = std::fclose(w.f) == 0;
+  if (rc != VLJ_OK) return rc;
+  if (!w.ok || !closed) return fail(sim, VLJ_E_INVALID, std::string("could not write ") + path);
+  return VLJ_OK;
+}
+
+int vlj_replica_info(const vlj_sim* sim, size_t replica, vlj_settings* settings, uint64_t* count,
+                     size_t* replicas) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (replicas) *replicas = sim->rep.size();
+  if (replica >= sim->rep.size()) return fail(nullptr, VLJ_E_INVALID, "replica out of range");
+  if (settings) *settings = sim->host[replica].settings;
+  if (count) *count = sim->rep[replica].count;
+  return VLJ_OK;
+}
+
+int vlj_load(const char* path, vlj_sim** out) {
+  if (!path || !out) return fail(nullptr, VLJ_E_INVALID, "path and out must not be NULL");
+  *out = nullptr;
+  FileIn r;
+  r.f = std::fopen(path, "rb");
+  if (!r.f) return fail(nullptr, VLJ_E_INVALID, std::string("could not open ") + path);
+  auto bad = [&](const std::string& why) {
+    std::fclose(r.f);
+    return fail(nullptr, VLJ_E_INVALID, std::string(path) + ": " + why);
+  };
+  StateHeader h{};
+  r.get(&h, sizeof h);
+  if (!r.ok || std::memcmp(h.magic, kStateMagic, 8) != 0) return bad("not a state file");
+  if (h.version != kStateVersion || h.sizeof_replica != sizeof(Replica) || h.sizeof_control != sizeof(Control) ||
+      h.sizeof_settings != sizeof(vlj_settings) || h.sizeof_brick != sizeof(Brick))
+    return bad("state file written by another version of the library");
+  if (h.replicas == 0 || h.replicas > (1u << 24)) return bad("bad replica count");
+  const size_t R = h.replicas;
+  std::vector<vlj_settings> settings(R);
+  std::vector<Replica> rep(R);
+  r.get(settings.data(), R * sizeof(vlj_settings));
+  r.get(rep.data(), R * sizeof(Replica));
+  if (!r.ok) return bad("truncated");
+  uint64_t atoms = 0, cells = 0;
+  for (const Replica& x : rep) {
+    if (x.atom_off != atoms || x.cell_off != cells) return bad("inconsistent replica table");
+    atoms += x.count;
+    cells += x.total_cells;
+  }
+  if (atoms != h.n_total || cells != h.cells_total) return bad("inconsistent sizes");
+  std::vector<uint64_t> prof_samples(R);
+  std::vector<std::vector<uint32_t>> bx(R), by(R);
+  for (size_t k = 0; k < R; k++) {
+    r.get(&prof_samples[k], sizeof(uint64_t));
+    if (!read_vec(r, bx[k], rep[k].cells[2]) || !read_vec(r, by[k], rep[k].cells[2])) return bad("truncated");
+  }
+  std::vector<Brick> bricks;
+  std::vector<Control> ctl;
+  StateArrays a;
+  const size_t N = h.n_total, C = h.cells_total, nb = h.n_bricks, ng = (nb + kCutGroup - 1) / kCutGroup;
+  bool ok = read_vec(r, bricks, nb) && read_vec(r, ctl, R);
+  for (int k = 0; k < 12 && ok; k++) ok = read_vec(r, a.f64[k], N);
+  ok = ok && read_vec(r, a.orig, N) && read_vec(r, a.cell_of, N) && read_vec(r, a.nbr_count, N) &&
+       read_vec(r, a.cell_start, C) && read_vec(r, a.cell_n, C) && read_vec(r, a.nbr, h.list_total) &&
+       read_vec(r, a.cut, nb) && read_vec(r, a.parts, nb * kHomeCells) && read_vec(r, a.cut_groups, ng) &&
+       read_vec(r, a.profile, h.prof_total) && read_vec(r, a.hc, (size_t)h.hc_cap * 3);
+  if (!ok) return bad("truncated");
+  std::fclose(r.f);
+  for (const Brick& g : bricks)
+    if (g.rep >= R || g.gx < 1 || g.gy < 1 || g.gx > (uint32_t)kBrickMax || g.gy > (uint32_t)kBrickMax ||
+        g.cell0 >= rep[g.rep].total_cells)
+      return fail(nullptr, VLJ_E_INVALID, std::string(path) + ": bad brick table");
+  // The particles in original order, to build the handle from.
+  std::vector<vlj_particle> parts(N);
+  std::vector<uint64_t> offsets(R + 1, 0);
+  for (size_t k = 0; k < R; k++) {
+    offsets[k + 1] = offsets[k] + rep[k].count;
+    for (uint32_t i = rep[k].atom_off; i < rep[k].atom_off + rep[k].count; i++) {
+      const uint32_t o = a.orig[i];
+      if (o >= rep[k].count) return fail(nullptr, VLJ_E_INVALID, std::string(path) + ": bad original index");
+      vlj_particle& p = parts[offsets[k] + o];
+      p.pos[0] = a.f64[0][i];
+      p.pos[1] = a.f64[1][i];
+      p.pos[2] = a.f64[2][i];
+      p.vel[0] = a.f64[3][i];
+      p.vel[1] = a.f64[4][i];
+      p.vel[2] = a.f64[5][i];
+    }
+  }
+  vlj_sim* s = nullptr;
+  int rc = create(parts.data(), offsets.data(), settings.data(), R, &s, rep.data());
+  if (rc != VLJ_OK) return rc;
+  auto reject = [&](int code, const std::string& why) {
+    vlj_destroy(s);
+    return fail(nullptr, code, std::string(path) + ": " + why);
+  };
+  if (s->n_total != N || s->cells_total != C || s->list_total != h.list_total || s->prof_total != h.prof_total)
+    return reject(VLJ_E_INVALID, "the settings give another layout than the saved one");
+  for (size_t k = 0; k < R; k++) {
+    const Replica& x = s->rep[k];
+    const Replica& y = rep[k];
+    if (x.count != y.count || x.total_cells != y.total_cells || x.capacity != y.capacity ||
+        x.list_off != y.list_off || x.prof_off != y.prof_off || x.cells[0] != y.cells[0] ||
+        x.cells[1] != y.cells[1] || x.cells[2] != y.cells[2] || x.atom_block_off != y.atom_block_off ||
+        x.scan_block_off != y.scan_block_off)
+      return reject(VLJ_E_INVALID, "replica " + std::to_string(k) + ": the settings give another layout than the saved one");
+  }
+  // The saved bricks replace the ones planned from the particles.
+  Buffers& B = s->B;
+  if (nb != s->bricks.size()) {
+    free_device(s, (void*)B.bricks);
+    free_device(s, B.cut);
+    free_device(s, B.parts);
+    free_device(s, B.cut_groups);
+    Brick* gb = nullptr;
+    rc = dev_alloc(s, &gb, nb);
+    if (rc == VLJ_OK) rc = dev_alloc(s, &B.cut, nb);
+    if (rc == VLJ_OK) rc = dev_alloc(s, &B.parts, nb * kHomeCells);
+    if (rc == VLJ_OK) rc = dev_alloc(s, &B.cut_groups, ng);
+    if (rc != VLJ_OK) return reject(rc, s->error);
+    B.bricks = gb;
+    B.n_bricks = (uint32_t)nb;
+  }
+  s->bricks = bricks;
+  s->rep = rep;
+  for (size_t k = 0; k < R; k++) {
+    s->host[k].prof_samples = prof_samples[k];
+    s->host[k].brick_x = bx[k];
+    s->host[k].brick_y = by[k];
+  }
+  s->any_heat_exchange = false;
+  for (const Replica& x : rep) s->any_heat_exchange = s->any_heat_exchange || x.hex_on;
+  s->steps = h.steps;
+  s->primed = h.primed != 0;
+  s->hc_samples = h.hc_samples;
+  if (h.hc_cap) {
+    hipError_t e = hipMalloc((void**)&B.hc, (size_t)h.hc_cap * 3 * sizeof(long long));
+    if (e != hipSuccess) return reject(VLJ_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  double* const* f = state_f64(B);
+  rc = upload(s, (Brick*)B.bricks, bricks);
+  if (rc == VLJ_OK) rc = upload(s, (Replica*)B.rep, rep);
+  if (rc == VLJ_OK) rc = upload(s, B.ctl, ctl);
+  for (int k = 0; k < 12 && rc == VLJ_OK; k++) rc = upload(s, f[k], a.f64[k]);
+  if (rc == VLJ_OK) rc = upload(s, B.orig, a.orig);
+  if (rc == VLJ_OK) rc = upload(s, B.cell_of, a.cell_of);
+  if (rc == VLJ_OK) rc = upload(s, B.nbr_count, a.nbr_count);
+  if (rc == VLJ_OK) rc = upload(s, B.cell_start, a.cell_start);
+  if (rc == VLJ_OK) rc = upload(s, B.cell_n, a.cell_n);
+  if (rc == VLJ_OK) rc = upload(s, B.nbr, a.nbr);
+  if (rc == VLJ_OK) rc = upload(s, B.cut, a.cut);
+  if (rc == VLJ_OK) rc = upload(s, B.parts, a.parts);
+  if (rc == VLJ_OK) rc = upload(s, B.cut_groups, a.cut_groups);
+  if (rc == VLJ_OK) rc = upload(s, B.profile, a.profile);
+  if (rc == VLJ_OK) rc = upload(s, B.hc, a.hc);
+  if (rc != VLJ_OK) return reject(rc, s->error);
+  *out = s;
   return VLJ_OK;
 }
 

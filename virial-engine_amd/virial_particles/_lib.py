@@ -110,6 +110,9 @@ SIGNATURES = {
     "vlj_read_heat_current_raw": (C.c_int, [_P, _i64p, _sz, _up]),
     "vlj_last_error": (C.c_char_p, [_P]),
     "vlj_batch_create": (C.c_int, [_pp, _u64p, C.POINTER(Settings), _sz, C.POINTER(_P)]),
+    "vlj_save": (C.c_int, [_P, C.c_char_p]),
+    "vlj_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "vlj_replica_info": (C.c_int, [_P, _sz, C.POINTER(Settings), _u64p, C.POINTER(_sz)]),
     "vlj_batch_run": (C.c_int, [_P, _u64]),
     "vlj_batch_measure_now": (C.c_int, [_P, C.POINTER(Measurement), _sz]),
     "vlj_batch_replicas": (_sz, [_P]),

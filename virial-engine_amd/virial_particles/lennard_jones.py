@@ -12,6 +12,7 @@ theorem, P = (2 KE + W) / (3 V) (lennard_jones.rs:110-131).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import enum
 import math
 from dataclasses import dataclass, field
@@ -116,6 +117,30 @@ class Settings:
         s.profile_bins = int(self.profile_bins)
         s.precision = int(self.precision)
         return s
+
+    @staticmethod
+    def from_c(s: L.Settings) -> "Settings":
+        t = s.thermostat
+        if t.kind == 1:
+            thermostat = Langevin(t.temperature, t.friction, int(t.seed))
+        elif t.kind == 2:
+            thermostat = NoseHoover(t.temperature, t.period)
+        else:
+            thermostat = None
+        return Settings(list(s.box_size), s.cutoff, s.dt, thermostat, bool(s.shifted_force), s.skin,
+                        int(s.profile_bins), int(s.precision))
+
+
+def _load_handle(path) -> C.c_void_p:
+    h = C.c_void_p()
+    _check(L.lib().vlj_load(os.fsencode(str(path)), C.byref(h)))
+    return h
+
+
+def _replica_info(h, replica: int):
+    st, n, reps = L.Settings(), C.c_uint64(), C.c_size_t()
+    _check(L.lib().vlj_replica_info(h, replica, C.byref(st), C.byref(n), C.byref(reps)), h)
+    return Settings.from_c(st), int(n.value), int(reps.value)
 
 
 @dataclass
@@ -332,6 +357,24 @@ class LennardJones:
         _check(L.lib().vlj_create(_pptr(arr), arr.shape[0], C.byref(cs), C.byref(h)))
         self._h = h
 
+    def save(self, path) -> None:
+        """Writes the complete state, after the steps taken so far, for an
+        exact resume with LennardJones.load (no counterpart in the
+        reference)."""
+        _check(L.lib().vlj_save(self._h, os.fsencode(str(path))), self._h)
+
+    @classmethod
+    def load(cls, path) -> "LennardJones":
+        """A handle from a file written by save(); it steps on to the same
+        bits as the handle that saved it."""
+        self = cls.__new__(cls)
+        self._h = _load_handle(path)
+        self._settings, _, reps = _replica_info(self._h, 0)
+        if reps != 1:
+            self.close()
+            raise LennardJonesError(L.E_INVALID, f"{path}: a batch state; load it with Batch.load")
+        return self
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             L.lib().vlj_destroy(self._h)
@@ -442,21 +485,25 @@ class LennardJones:
         return flux.value if has.value else None
 
     def record_heat_current(self, capacity: int) -> None:
-        self._hc_capacity = int(capacity)
-        self._c(L.lib().vlj_record_heat_current(self._h, self._hc_capacity))
+        self._c(L.lib().vlj_record_heat_current(self._h, int(capacity)))
 
     def sample_heat_current(self) -> None:
         self._c(L.lib().vlj_sample_heat_current(self._h))
 
+    def _hc_samples(self) -> int:
+        n = C.c_uint32()
+        self._c(L.lib().vlj_read_heat_current_raw(self._h, None, 0, C.byref(n)))
+        return n.value
+
     def read_heat_current(self) -> List[List[float]]:
-        cap = max(getattr(self, "_hc_capacity", 0), 1)
+        cap = max(self._hc_samples(), 1)
         out = np.zeros(3 * cap)
         n = C.c_uint32()
         self._c(L.lib().vlj_read_heat_current(self._h, _dptr(out), cap, C.byref(n)))
         return out[: 3 * n.value].reshape(-1, 3).tolist()
 
     def read_heat_current_raw(self) -> np.ndarray:
-        cap = max(getattr(self, "_hc_capacity", 0), 1)
+        cap = max(self._hc_samples(), 1)
         out = np.zeros((cap, 3), dtype=np.int64)
         n = C.c_uint32()
         self._c(L.lib().vlj_read_heat_current_raw(self._h, out.ctypes.data_as(C.POINTER(C.c_int64)), cap,
@@ -552,6 +599,22 @@ class Batch:
         self._h = h
         self.replicas = len(replicas)
         self.profile_bins = [int(st.profile_bins) for _, st in replicas]
+
+    def save(self, path) -> None:
+        """Writes the complete state of every replica (see LennardJones.save)."""
+        _check(L.lib().vlj_save(self._h, os.fsencode(str(path))), self._h)
+
+    @classmethod
+    def load(cls, path) -> "Batch":
+        """A batch from a file written by save()."""
+        self = cls.__new__(cls)
+        self._h = _load_handle(path)
+        _, _, reps = _replica_info(self._h, 0)
+        info = [_replica_info(self._h, r) for r in range(reps)]
+        self.sizes = [n for _, n, _ in info]
+        self.replicas = reps
+        self.profile_bins = [int(st.profile_bins) for st, _, _ in info]
+        return self
 
     def close(self) -> None:
         if getattr(self, "_h", None):
