@@ -193,6 +193,24 @@ int vlj_batch_run(vlj_batch *batch, uint64_t steps);
 /* One measurement per replica, in replica order. */
 int vlj_batch_measure_now(vlj_batch *batch, vlj_measurement *per_replica, size_t replicas);
 
+/* ---- Renderer bindings: LennardJones::positions / velocities / set_drawn
+ * (rs:1041-1051, :885-889; lennard_jones.wgsl:596-603). */
+/* Device pointers to the render arrays, `count` float4 each in slot order
+ * (the device's order of the same particles): the position wrapped into the
+ * box (w 0), and the velocity with in w the value chosen by vlj_set_drawn.
+ * Allocated on the first call, valid until the handle is destroyed; filled
+ * by vlj_render and by every measurement. */
+int vlj_render_buffers(vlj_sim *sim, void **positions, void **velocities);
+/* Fills the render arrays from the current state, ordered after the steps
+ * enqueued so far (asynchronous). */
+int vlj_render(vlj_sim *sim);
+/* vlj_render, then copies both arrays (count float4 each) to host memory. */
+int vlj_read_render(vlj_sim *sim, float *positions, float *velocities, size_t count);
+/* set_drawn: 0 neighbours within 1.5 sigma (the default), 1 local
+ * temperature, m v^2 / 3 averaged over the molecule and its neighbours
+ * within the cutoff. VLJ_E_INVALID for another value. */
+int vlj_set_drawn(vlj_sim *sim, uint32_t drawn);
+
 /* ---- Checkpoint and exact resume (no counterpart in the reference, whose
  * runs start again from particles; the on-disk formats of its app crate,
  * crates/virial/src/data.rs, hold measurements and snapshots only). */

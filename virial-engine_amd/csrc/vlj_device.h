@@ -80,6 +80,7 @@ struct Replica {
   double bin_width;    // profile slab height, L_z / prof_bins
   uint32_t prof_bins;  // profile slabs along z (0: no profiles)
   uint32_t prof_off;   // first of the replica's prof_bins * 8 + 3 profile sums
+  uint32_t drawn;      // render w of velocities: 0 neighbours within 1.5, 1 local temperature
 };
 
 // Control words of one replica (lennard_jones.wgsl:106-110: MAX_DISPLACEMENT,
@@ -138,6 +139,7 @@ struct Buffers {
   uint16_t* nbr;                                             // lists of staged indices
   long long* slots;                                          // kSlots per replica
   long long* profile;                                        // profile sums of all replicas
+  float *render_pos, *render_vel;                            // float4 per slot, for drawing
   uint32_t* tickets;                                         // last-block counters
 };
 

@@ -1592,6 +1592,73 @@ __global__ void __launch_bounds__(kBrickThreads) k_heat_current(Buffers B, uint3
   }
 }
 
+// ---------------------------------------------------------------------------
+// Render arrays (lennard_jones.rs:1041-1051, lennard_jones.wgsl:596-603):
+// float4 per slot, the position wrapped into the box (w 0) and the velocity
+// with, in w, the neighbours within 1.5 sigma or (drawn 1) m v^2 / 3
+// averaged over the molecule and its neighbours within the cutoff.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+                                            bool staged) {
+  const uint32_t H = S.hoff[g.gx * g.gy];
+  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
+    const Home at = home_atom(g, S, h);
+    const uint32_t i = at.slot;
+    double xi, yi, zi;
+    staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
+    const double v[3] = {B.vx[i], B.vy[i], B.vz[i]};
+    uint32_t near = 0, within = 0;
+    double v2 = 0.0;
+    const uint32_t cnt = B.nbr_count[i];
+    const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
+    for (uint32_t k = 0; k < cnt; k++) {
+      const uint32_t e = lst[list_entry(k)];
+      double xj, yj, zj;
+      staged_pos(R, S, staged, e, B.x, B.y, B.z, xj, yj, zj);
+      const double r2 = dist2(xj - xi, yj - yi, zj - zi);
+      if (!(r2 < R.rc2)) continue;
+      within++;
+      near += r2 < kNearR2 ? 1u : 0u;
+      if (R.drawn == 1u) {
+        const uint32_t c = stage_cell_of(S, e);
+        const uint32_t j = S.start[c] + (e - S.off[c]);
+        v2 += dist2(B.vx[j], B.vy[j], B.vz[j]);
+      }
+    }
+    const float w = R.drawn == 1u ? (float)((dist2(v[0], v[1], v[2]) + v2) / (3.0 * (double)(within + 1u)))
+                                  : (float)near;
+    float4* P = reinterpret_cast<float4*>(B.render_pos);
+    float4* V = reinterpret_cast<float4*>(B.render_vel);
+    P[i] = make_float4((float)wrap_one(xi, R.box[0]), (float)wrap_one(yi, R.box[1]), (float)wrap_one(zi, R.box[2]),
+                       0.0f);
+    V[i] = make_float4((float)v[0], (float)v[1], (float)v[2], w);
+  }
+}
+
+template <bool PARTS>
+__global__ void __launch_bounds__(kBrickThreads) k_render(Buffers B) {
+  __shared__ Stage S;
+  if constexpr (!PARTS) {
+    const uint32_t gb = dealt_index(blockIdx.x);
+    if (gb >= B.n_bricks) return;
+    const Brick g = B.bricks[gb];
+    const Replica R = B.rep[g.rep];
+    const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+    if (!staged && g.gx * g.gy > 1u) return;
+    render_part(B, R, S, g, staged);
+  } else {
+    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+      const Brick brick = B.bricks[gb];
+      const Replica R = B.rep[brick.rep];
+      const Brick g = part_brick(brick, code, R);
+      __syncthreads();  // the previous part is done with S
+      const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
+      render_part(B, R, S, g, staged);
+    });
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_zero_slots(Buffers B) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < B.n_replicas * kSlots) B.slots[t] = 0;
@@ -1667,6 +1734,13 @@ hipError_t launch_heat_exchange(const Buffers& B, hipStream_t s) {
 
 // Grid of a launch over the bricks that are cut into parts.
 inline dim3 parts_grid(const Buffers& B) { return grid1(std::min<uint32_t>(B.n_bricks, kPartsGrid)); }
+
+hipError_t launch_render(const Buffers& B, hipStream_t s) {
+  if (!B.n_bricks || !B.render_pos) return hipSuccess;
+  hipLaunchKernelGGL(k_render<false>, grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
+  hipLaunchKernelGGL(k_render<true>, parts_grid(B), dim3(kBrickThreads), 0, s, B);
+  return hipGetLastError();
+}
 
 hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, hipStream_t s) {
   if (!B.n_bricks) return hipSuccess;

@@ -675,6 +675,7 @@ int enqueue_measure(vlj_sim* s, int slot) {
   Ring& r = s->ring[slot];
   const size_t R = s->rep.size();
   HIP_TRY(s, launch_measure(s->B, s->precision, s->shifted, s->stream));
+  HIP_TRY(s, launch_render(s->B, s->stream));  // the drawn values, as a measurement leaves them
   HIP_TRY(s, hipMemcpyAsync(r.slots, s->B.slots, R * kSlots * sizeof(long long), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(s, hipMemcpyAsync(r.ctl, s->B.ctl, R * sizeof(Control), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(s, hipEventRecord(r.done, s->stream));
@@ -1650,6 +1651,55 @@ int vlj_save(vlj_sim* sim, const char* path) {
   const bool closed = std::fclose(w.f) == 0;
   if (rc != VLJ_OK) return rc;
   if (!w.ok || !closed) return fail(sim, VLJ_E_INVALID, std::string("could not write ") + path);
+  return VLJ_OK;
+}
+
+int vlj_render_buffers(vlj_sim* sim, void** positions, void** velocities) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (!sim->B.render_pos) {
+    int rc = dev_alloc(sim, &sim->B.render_pos, (size_t)sim->n_total * 4);
+    if (rc == VLJ_OK) rc = dev_alloc(sim, &sim->B.render_vel, (size_t)sim->n_total * 4);
+    if (rc != VLJ_OK) return rc;
+    HIP_TRY(sim, hipMemsetAsync(sim->B.render_pos, 0, std::max<size_t>(sim->n_total, 1) * 16, sim->stream));
+    HIP_TRY(sim, hipMemsetAsync(sim->B.render_vel, 0, std::max<size_t>(sim->n_total, 1) * 16, sim->stream));
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  }
+  if (positions) *positions = sim->B.render_pos;
+  if (velocities) *velocities = sim->B.render_vel;
+  return VLJ_OK;
+}
+
+int vlj_render(vlj_sim* sim) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  int rc = vlj_render_buffers(sim, nullptr, nullptr);
+  if (rc == VLJ_OK) rc = prime_if_needed(sim);
+  if (rc != VLJ_OK) return rc;
+  HIP_TRY(sim, launch_render(sim->B, sim->stream));
+  return VLJ_OK;
+}
+
+int vlj_read_render(vlj_sim* sim, float* positions, float* velocities, size_t count) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (count < sim->n_total) return fail(sim, VLJ_E_INVALID, "the output holds fewer particles than the handle");
+  int rc = vlj_render(sim);
+  if (rc != VLJ_OK) return rc;
+  const size_t bytes = (size_t)sim->n_total * 16;
+  if (positions && bytes) HIP_TRY(sim, hipMemcpyAsync(positions, sim->B.render_pos, bytes, hipMemcpyDeviceToHost, sim->stream));
+  if (velocities && bytes) HIP_TRY(sim, hipMemcpyAsync(velocities, sim->B.render_vel, bytes, hipMemcpyDeviceToHost, sim->stream));
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  return VLJ_OK;
+}
+
+int vlj_set_drawn(vlj_sim* sim, uint32_t drawn) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  if (drawn > 1u) return fail(sim, VLJ_E_INVALID, "drawn is 0 (neighbours) or 1 (temperature)");
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  for (size_t r = 0; r < sim->rep.size(); r++) {
+    sim->rep[r].drawn = drawn;
+    HIP_TRY(sim, hipMemcpyAsync((void*)&sim->B.rep[r], &sim->rep[r], sizeof(Replica), hipMemcpyHostToDevice,
+                                sim->stream));
+  }
+  HIP_TRY(sim, hipStreamSynchronize(sim->stream));
   return VLJ_OK;
 }
 

@@ -111,6 +111,10 @@ SIGNATURES = {
     "vlj_last_error": (C.c_char_p, [_P]),
     "vlj_batch_create": (C.c_int, [_pp, _u64p, C.POINTER(Settings), _sz, C.POINTER(_P)]),
     "vlj_save": (C.c_int, [_P, C.c_char_p]),
+    "vlj_render_buffers": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "vlj_render": (C.c_int, [_P]),
+    "vlj_read_render": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), _sz]),
+    "vlj_set_drawn": (C.c_int, [_P, C.c_uint32]),
     "vlj_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "vlj_replica_info": (C.c_int, [_P, _sz, C.POINTER(Settings), _u64p, C.POINTER(_sz)]),
     "vlj_batch_run": (C.c_int, [_P, _u64]),

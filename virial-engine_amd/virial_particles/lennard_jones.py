@@ -195,6 +195,45 @@ class Slab:
     potential: float       # half of each pair's energy to each particle's slab
 
 
+class Drawn(enum.IntEnum):
+    """What measurements and render() leave in the w of each velocity
+    (lennard_jones.rs:365-373)."""
+
+    NEIGHBOURS = 0  # neighbours within 1.5 sigma, a local density (the default)
+    TEMPERATURE = 1  # m v^2 / 3 over the molecule and its neighbours within the cutoff
+
+
+def _render_methods(cls):
+    """positions()/velocities() for drawing (lennard_jones.rs:1041-1051, 885-889)."""
+
+    def set_drawn(self, drawn: Drawn) -> None:
+        _check(L.lib().vlj_set_drawn(self._h, int(drawn)), self._h)
+
+    def render(self) -> None:
+        """Fills the float4 render arrays on the device (asynchronous)."""
+        _check(L.lib().vlj_render(self._h), self._h)
+
+    def render_buffers(self):
+        """Device addresses of the float4 render arrays (positions,
+        velocities), one float4 per particle in slot order."""
+        p, v = C.c_void_p(), C.c_void_p()
+        _check(L.lib().vlj_render_buffers(self._h, C.byref(p), C.byref(v)), self._h)
+        return p.value, v.value
+
+    def read_render(self):
+        """Host copies of the render arrays, (n, 4) float32 each, slot order."""
+        n = self.count() if hasattr(self, "count") else sum(self.sizes)
+        pos = np.zeros((n, 4), dtype=np.float32)
+        vel = np.zeros((n, 4), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        _check(L.lib().vlj_read_render(self._h, pos.ctypes.data_as(fp), vel.ctypes.data_as(fp), n), self._h)
+        return pos, vel
+
+    for f in (set_drawn, render, render_buffers, read_render):
+        setattr(cls, f.__name__, f)
+    return cls
+
+
 class Phase(enum.Enum):
     """Where a slab sits in a liquid-vapour profile (lennard_jones.rs:149-156)."""
 
@@ -341,6 +380,7 @@ def to_particles(a: np.ndarray) -> List[Particle]:
     return [Particle(list(map(float, r["pos"])), list(map(float, r["vel"]))) for r in a]
 
 
+@_render_methods
 class LennardJones:
     """Lennard-Jones particles on the GPU (lennard_jones.rs:332-1375).
 
@@ -582,6 +622,7 @@ def _debug_counters(h) -> dict:
             "bricks": int(out[3]), "global_cell_builds": int(out[4])}
 
 
+@_render_methods
 class Batch:
     """Many independent replicas in the same launches (no counterpart in the
     reference). `replicas` is a list of (particles, Settings)."""
