@@ -141,6 +141,10 @@ struct Buffers {
   long long* profile;                                        // profile sums of all replicas
   float *render_pos, *render_vel;                            // float4 per slot, for drawing
   uint32_t* tickets;                                         // last-block counters
+  // flags[0]: replicas rebuilding this step (zeroed by the drift, counted by
+  // the decision); flags[1]: bricks cut into parts. Kernels that have
+  // nothing to do without them return after one load.
+  uint32_t* flags;
 };
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on

@@ -80,6 +80,7 @@ __device__ void decide_from(const Buffers& B, uint32_t r, double moved2, bool ad
   if (rebuild) {
     c->rebuilds = rebuilds + 1u;
     c->built = 1u;
+    atomicAdd(&B.flags[0], 1u);
   }
   if (advance_step) c->step = c->step + 1ull;
 }
@@ -113,6 +114,7 @@ __global__ void __launch_bounds__(kBlock) k_decide(Buffers B, int advance) {
 // LANGEVIN: some replica of the launch uses BAOAB (others stay NVE).
 template <bool LANGEVIN>
 __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) B.flags[0] = 0u;  // counted again by k_decide
   const uint32_t b = dealt_index(blockIdx.x);
   if (b >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[b];
@@ -388,6 +390,7 @@ __device__ __forceinline__ uint32_t run_atomic_add(uint32_t* counter, uint32_t k
 }
 
 __global__ void __launch_bounds__(kBlock) k_bin(Buffers B) {
+  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
   const uint32_t ab = dealt_index(blockIdx.x);
   if (ab >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[ab];
@@ -411,6 +414,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(Buffers B) {
 // Exclusive scan of counts inside each block of 1024 cells (4 per thread);
 // the last workgroup scans the block totals of each rebuilding replica.
 __global__ void __launch_bounds__(kBlock) k_scan(Buffers B) {
+  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
   const Range blk = B.scan_blocks[blockIdx.x];
   const bool active = B.ctl[blk.rep].rebuild != 0u;
   __shared__ uint32_t wsum[kBlock / 64];
@@ -464,6 +468,7 @@ __device__ __forceinline__ uint32_t cell_first_slot(const Buffers& B, const Repl
 }
 
 __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
+  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
   const uint32_t ab = dealt_index(blockIdx.x);
   if (ab >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[ab];
@@ -480,6 +485,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
 // in the cell, which puts the cell in ascending slot order whatever order
 // the scatter produced (the result of grid.wgsl:129-148's insertion sort).
 __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
+  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
   const uint32_t sb = dealt_index(blockIdx.x);
   if (sb >= B.n_sort_blocks) return;
   const Range blk = B.sort_blocks[sb];
@@ -514,6 +520,7 @@ __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
 // Slot t takes the atom at sorted[t] (lennard_jones.wgsl:419-452); the copy
 // goes to the second state set and `build` copies it back.
 __global__ void __launch_bounds__(kBlock) k_reorder(Buffers B) {
+  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
   const uint32_t ab = dealt_index(blockIdx.x);
   if (ab >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[ab];
@@ -805,6 +812,7 @@ __device__ __forceinline__ void each_cut_part(const Buffers& B, F&& f) {
   __shared__ uint32_t glist[kBrickThreads];  // groups with cut bricks, of this round
   __shared__ uint32_t blist[kBrickThreads];  // cut bricks of the current group
   __shared__ uint32_t first[kBrickThreads];  // number of each one's first part within the group
+  if (B.flags[1] == 0u) return;  // no brick is cut
   const uint32_t t = threadIdx.x, G = gridDim.x;
   const uint32_t groups = (B.n_bricks + kCutGroup - 1u) / kCutGroup;
   uint32_t base = 0;  // parts of the groups before
@@ -865,6 +873,7 @@ __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
 constexpr uint32_t kPiece = VLJ_BUILD_PIECE;
 constexpr int kGroup = VLJ_BUILD_GROUP;
 static_assert(kPiece >= 1 && kPiece <= 64, "a lane per home atom of a work item");
+static_assert(kListStride == 64, "entry pos of a list lies (pos / 8) KB + 2 (pos % 8) bytes from its base");
 
 // One workgroup per brick; its waves build the lists of one work item (a
 // piece of a home cell's atoms) at a time. A home cell's candidates are the
@@ -989,17 +998,23 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
         double xi, yi, zi;
         staged_pos(R, S, staged, e, X, Y, Z, xi, yi, zi);
         uint32_t f = readlane_u32(found, k - k0);
-        uint16_t* lst = B.nbr + list_base(R, lbase + k);
+        char* lst = (char*)(B.nbr + list_base(R, lbase + k));
+        // the atom itself is candidate number e - rdelta[4] (the home cell is
+        // the middle of run 4, dz = dy = 1): its lane in chunk c, if any
+        const uint32_t self = e - rdelta[4] - c0;
 #pragma unroll
         for (int c = 0; c < kGroup; c++) {
           if ((uint32_t)c < nc) {
             // a ballot marks the candidates within rc + skin (not the atom
             // itself); each hit goes at the list length plus its rank
-            const bool hit = q[c] != e && dist2(cx[c] - xi, cy[c] - yi, cz[c] - zi) < R.list_r2;
-            const unsigned long long mask = __builtin_amdgcn_ballot_w64(hit);
+            const bool near = dist2(cx[c] - xi, cy[c] - yi, cz[c] - zi) < R.list_r2;
+            const uint32_t sl = self - 64u * (uint32_t)c;  // the atom's own lane, if below 64
+            const unsigned long long mask =
+                __builtin_amdgcn_ballot_w64(near) & ~(sl < 64u ? 1ull << sl : 0ull);
             if (mask != 0ull) {
               const uint32_t pos = f + lane_rank(mask);
-              if (hit && pos < cap) lst[list_entry(pos)] = (uint16_t)q[c];
+              if (near && lane != sl && pos < cap)
+                *(uint16_t*)(lst + (size_t)(((pos & ~7u) << 7) | ((pos & 7u) << 1))) = (uint16_t)q[c];
               f += (uint32_t)__builtin_popcountll(mask);
             }
           }
@@ -1029,6 +1044,7 @@ template <bool PARTS>
 __global__ void __launch_bounds__(kBrickThreads) k_build(Buffers B) {
   __shared__ Stage S;
   __shared__ uint32_t next_item;
+  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
   if constexpr (!PARTS) {
     const uint32_t gb = dealt_index(blockIdx.x);
     if (gb >= B.n_bricks) return;
@@ -1047,7 +1063,10 @@ __global__ void __launch_bounds__(kBrickThreads) k_build(Buffers B) {
       const uint32_t was = B.cut[gb];
       const uint32_t np = cut ? cut_brick(g, S, B.parts + (size_t)gb * kHomeCells) : 0u;
       B.cut[gb] = np;
-      if ((was != 0u) != (np != 0u)) atomicAdd(&B.cut_groups[gb / kCutGroup], np != 0u ? 1u : 0xffffffffu);
+      if ((was != 0u) != (np != 0u)) {
+        atomicAdd(&B.cut_groups[gb / kCutGroup], np != 0u ? 1u : 0xffffffffu);
+        atomicAdd(&B.flags[1], np != 0u ? 1u : 0xffffffffu);
+      }
       if (cut) atomicAdd(&B.ctl[g.rep].stage_overflow, 1u);
     }
     if (cut) return;

@@ -547,6 +547,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   A(&B.slots, replicas * kSlots);
   A(&B.profile, std::max<uint64_t>(s->prof_total, 1));
   A(&B.tickets, 4);
+  A(&B.flags, 2);
   A(&s->fixed_scratch, kSlots);
   if (rc != VLJ_OK) {
     std::string msg = s->error;
@@ -598,6 +599,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   if (e == hipSuccess) e = hipMemset(B.cell_count, 0, std::max<size_t>(C, 1) * 4);
   if (e == hipSuccess) e = hipMemset(B.cell_cursor, 0, std::max<size_t>(C, 1) * 4);
   if (e == hipSuccess) e = hipMemset(B.tickets, 0, 16);
+  if (e == hipSuccess) e = hipMemset(B.flags, 0, 8);
   if (e == hipSuccess) e = hipMemset(B.ax, 0, std::max<size_t>(N, 1) * 8);
   if (e == hipSuccess) e = hipMemset(B.ay, 0, std::max<size_t>(N, 1) * 8);
   if (e == hipSuccess) e = hipMemset(B.az, 0, std::max<size_t>(N, 1) * 8);
@@ -1846,6 +1848,11 @@ int vlj_load(const char* path, vlj_sim** out) {
   if (rc == VLJ_OK) rc = upload(s, B.cut, a.cut);
   if (rc == VLJ_OK) rc = upload(s, B.parts, a.parts);
   if (rc == VLJ_OK) rc = upload(s, B.cut_groups, a.cut_groups);
+  if (rc == VLJ_OK) {
+    uint32_t n_cut = 0;
+    for (uint32_t c : a.cut) n_cut += c != 0u ? 1u : 0u;
+    rc = upload(s, B.flags + 1, std::vector<uint32_t>{n_cut});
+  }
   if (rc == VLJ_OK) rc = upload(s, B.profile, a.profile);
   if (rc == VLJ_OK) rc = upload(s, B.hc, a.hc);
   if (rc != VLJ_OK) return reject(rc, s->error);
