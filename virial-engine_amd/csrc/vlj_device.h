@@ -24,7 +24,10 @@ constexpr int kBlock = 256;       // threads per atom block (4 wave64)
 constexpr int kBrickThreads = VLJ_BRICK_THREADS;  // threads of a brick workgroup
 static_assert(kBrickThreads % 64 == 0 && kBrickThreads <= 1024, "whole waves");
 constexpr int kScanCells = 1024;  // cells per scan block (256 threads x 4)
-constexpr int kSortCells = 4;     // cells per sort block (one wave64 per cell)
+constexpr int kSortCells = 4;     // cells per sort block of a dense replica (a wave64 per cell)
+constexpr int kSortCellsSparse = 256;  // ... of a replica with fewer than kSortDense atoms per cell
+constexpr double kSortDense = 6.0;
+constexpr uint32_t kSmallCell = 8;  // a cell this small is ranked by one thread
 constexpr uint32_t kProfileSums = 8;       // sums per profile slab (lennard_jones.wgsl:644)
 constexpr uint32_t kMaxProfileBins = 1u << 20;  // profile slabs per replica, at most
 constexpr uint32_t kHexSums = 18;   // per slab (hot, cold): count, momentum x3, v^2, f.v, force x3

@@ -453,9 +453,17 @@ __global__ void __launch_bounds__(kBlock) k_scan(Buffers B) {
       if (!B.ctl[r].rebuild) continue;
       const Replica R = B.rep[r];
       uint32_t acc = 0;
-      for (uint32_t b = R.scan_block_off; b < R.scan_block_off + R.scan_blocks; b++) {
-        B.bprefix[b] = acc;
-        acc += load_agent(B.bsum + b);
+      // 16 block sums in flight at a time (a dilute replica has tens of blocks)
+      for (uint32_t b0 = 0; b0 < R.scan_blocks; b0 += 16u) {
+        uint32_t v[16];
+#pragma unroll
+        for (uint32_t k = 0; k < 16u; k++)
+          v[k] = b0 + k < R.scan_blocks ? load_agent(B.bsum + R.scan_block_off + b0 + k) : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 16u; k++) {
+          if (b0 + k < R.scan_blocks) B.bprefix[R.scan_block_off + b0 + k] = acc;
+          acc += v[k];
+        }
       }
     }
     if (threadIdx.x == 0) atomicExch(B.tickets + 1, 0u);
@@ -484,6 +492,11 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
 // One wave64 per cell: every entry's rank is the number of smaller entries
 // in the cell, which puts the cell in ascending slot order whatever order
 // the scatter produced (the result of grid.wgsl:129-148's insertion sort).
+// Per-cell order (R3/R8): each cell's slots in ascending original slot,
+// by rank (the number of smaller entries). A sort block covers kSortCells
+// cells of a dense replica (a wave per cell) or kSortCellsSparse of a dilute
+// one: there a thread ranks a cell of at most kSmallCell atoms on its own,
+// and the waves take the larger cells of the block in turn.
 __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
   if (B.flags[0] == 0u) return;  // no replica rebuilds this step
   const uint32_t sb = dealt_index(blockIdx.x);
@@ -491,29 +504,54 @@ __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
   const Range blk = B.sort_blocks[sb];
   if (!B.ctl[blk.rep].rebuild) return;
   const Replica R = B.rep[blk.rep];
-  const uint32_t c = blk.first + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (c >= blk.end) return;
-  const uint32_t start = cell_first_slot(B, R, c);
-  const uint32_t n = B.cell_count[c];
-  if (n <= 64u) {
-    uint32_t e = lane < (int)n ? B.unsorted[start + lane] : 0xffffffffu;
-    uint32_t rank = 0;
-    for (uint32_t m = 0; m < n; m++) rank += (__shfl(e, (int)m, 64) < e) ? 1u : 0u;
-    if (lane < (int)n) B.sorted[start + rank] = e;
-  } else {
-    for (uint32_t k = (uint32_t)lane; k < n; k += 64u) {
-      uint32_t e = B.unsorted[start + k];
-      uint32_t rank = 0;
-      for (uint32_t m = 0; m < n; m++) rank += (B.unsorted[start + m] < e) ? 1u : 0u;
-      B.sorted[start + rank] = e;
-    }
-  }
-  if (lane == 0) {
+  __shared__ uint32_t big_c[kBlock], big_start[kBlock], big_n[kBlock];
+  __shared__ uint32_t n_big;
+  if (threadIdx.x == 0) n_big = 0u;
+  __syncthreads();
+  const uint32_t c = blk.first + threadIdx.x;
+  if (c < blk.end) {
+    const uint32_t start = cell_first_slot(B, R, c);
+    const uint32_t n = B.cell_count[c];
     B.cell_start[c] = start;
     B.cell_n[c] = n;
     B.cell_count[c] = 0u;
     B.cell_cursor[c] = 0u;
+    if (n <= kSmallCell) {
+      uint32_t e[kSmallCell];
+#pragma unroll
+      for (uint32_t m = 0; m < kSmallCell; m++) e[m] = m < n ? B.unsorted[start + m] : 0xffffffffu;
+#pragma unroll
+      for (uint32_t m = 0; m < kSmallCell; m++) {
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kSmallCell; j++) rank += e[j] < e[m] ? 1u : 0u;
+        if (m < n) B.sorted[start + rank] = e[m];
+      }
+    } else {
+      const uint32_t k = atomicAdd(&n_big, 1u);
+      big_c[k] = c;
+      big_start[k] = start;
+      big_n[k] = n;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const uint32_t nb = n_big;
+  for (uint32_t k = threadIdx.x >> 6; k < nb; k += kBlock / 64) {
+    const uint32_t start = big_start[k], n = big_n[k];
+    if (n <= 64u) {
+      uint32_t e = lane < (int)n ? B.unsorted[start + lane] : 0xffffffffu;
+      uint32_t rank = 0;
+      for (uint32_t m = 0; m < n; m++) rank += (__shfl(e, (int)m, 64) < e) ? 1u : 0u;
+      if (lane < (int)n) B.sorted[start + rank] = e;
+    } else {
+      for (uint32_t j = (uint32_t)lane; j < n; j += 64u) {
+        uint32_t e = B.unsorted[start + j];
+        uint32_t rank = 0;
+        for (uint32_t m = 0; m < n; m++) rank += (B.unsorted[start + m] < e) ? 1u : 0u;
+        B.sorted[start + rank] = e;
+      }
+    }
   }
 }
 
@@ -1183,7 +1221,10 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
   }
 }
 
-constexpr int kOctetBatch = 4;  // list octets (32 entries) in flight per batch
+#ifndef VLJ_OCTET_BATCH
+#define VLJ_OCTET_BATCH 4
+#endif
+constexpr int kOctetBatch = VLJ_OCTET_BATCH;  // list octets (8 entries each) in flight per batch
 
 __device__ __forceinline__ u32x4 load_octet(const uint16_t* base, uint32_t o, uint32_t no) {
   if (o < no) return __builtin_nontemporal_load((const u32x4*)(base + (uint64_t)o * (8u * kListStride)));

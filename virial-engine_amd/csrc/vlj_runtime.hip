@@ -481,9 +481,9 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     for (uint32_t b = 0; b < R.total_cells; b += kScanCells)
       s->scan_blocks.push_back(
           {(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + kScanCells), 0});
-    for (uint32_t b = 0; b < R.total_cells; b += kSortCells)
-      s->sort_blocks.push_back(
-          {(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + kSortCells), 0});
+    const uint32_t per = (double)R.count >= kSortDense * (double)R.total_cells ? kSortCells : kSortCellsSparse;
+    for (uint32_t b = 0; b < R.total_cells; b += per)
+      s->sort_blocks.push_back({(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + per), 0});
     atom_off += R.count;
     cell_off += R.total_cells;
     scan_off += R.scan_blocks;
