@@ -23,7 +23,8 @@ constexpr int kBlock = 256;       // threads per atom block (4 wave64)
 #endif
 constexpr int kBrickThreads = VLJ_BRICK_THREADS;  // threads of a brick workgroup
 static_assert(kBrickThreads % 64 == 0 && kBrickThreads <= 1024, "whole waves");
-constexpr int kScanCells = 1024;  // cells per scan block (256 threads x 4)
+constexpr int kScanCells = 1024;  // cells per scan block (256 threads x 4) of a dense replica
+constexpr int kScanCellsSparse = 4096;  // ... of a dilute one (256 threads x 16): fewer block sums
 constexpr int kSortCells = 4;     // cells per sort block of a dense replica (a wave64 per cell)
 constexpr int kSortCellsSparse = 256;  // ... of a replica with fewer than kSortDense atoms per cell
 constexpr double kSortDense = 6.0;
@@ -63,7 +64,7 @@ struct Replica {
   uint32_t cells[3];
   uint32_t atom_off, count;
   uint32_t cell_off, total_cells;
-  uint32_t scan_block_off, scan_blocks;
+  uint32_t scan_block_off, scan_blocks, scan_cells;  // scan blocks of scan_cells cells
   uint32_t atom_block_off, atom_blocks;  // this replica's entries of the atom-block table
   uint32_t capacity;
   uint32_t thermostat;  // 0 none, 1 Langevin (BAOAB), 2 Nose-Hoover

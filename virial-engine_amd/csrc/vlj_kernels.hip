@@ -419,13 +419,16 @@ __global__ void __launch_bounds__(kBlock) k_scan(Buffers B) {
   const bool active = B.ctl[blk.rep].rebuild != 0u;
   __shared__ uint32_t wsum[kBlock / 64];
   if (active) {
-    const uint32_t base = blk.first + threadIdx.x * 4u;
-    uint32_t v[4];
+    // each thread scans `per` consecutive cells (4 or 16, by the block's size)
+    const uint32_t per = (blk.end - blk.first + kBlock - 1u) / kBlock;
+    const uint32_t base = blk.first + threadIdx.x * per;
+    constexpr int kMaxPer = kScanCellsSparse / kBlock;
+    uint32_t v[kMaxPer];
     uint32_t local = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < kMaxPer; k++) {
       uint32_t c = base + (uint32_t)k;
-      v[k] = c < blk.end ? B.cell_count[c] : 0u;
+      v[k] = (uint32_t)k < per && c < blk.end ? B.cell_count[c] : 0u;
       local += v[k];
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -441,9 +444,9 @@ __global__ void __launch_bounds__(kBlock) k_scan(Buffers B) {
     }
     uint32_t run = before + incl - local;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < kMaxPer; k++) {
       uint32_t c = base + (uint32_t)k;
-      if (c < blk.end) B.cell_scan[c] = run;
+      if ((uint32_t)k < per && c < blk.end) B.cell_scan[c] = run;
       run += v[k];
     }
     if (threadIdx.x == 0) B.bsum[blockIdx.x] = total;
@@ -471,7 +474,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(Buffers B) {
 }
 
 __device__ __forceinline__ uint32_t cell_first_slot(const Buffers& B, const Replica& R, uint32_t c) {
-  uint32_t sb = R.scan_block_off + (c - R.cell_off) / (uint32_t)kScanCells;
+  uint32_t sb = R.scan_block_off + (c - R.cell_off) / R.scan_cells;
   return R.atom_off + B.cell_scan[c] + B.bprefix[sb];
 }
 

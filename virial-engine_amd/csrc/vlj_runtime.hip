@@ -458,7 +458,9 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     R.atom_off = atom_off;
     R.cell_off = cell_off;
     R.scan_block_off = scan_off;
-    R.scan_blocks = (R.total_cells + kScanCells - 1) / kScanCells;
+    const bool dense = (double)R.count >= kSortDense * (double)R.total_cells;
+    R.scan_cells = dense ? kScanCells : kScanCellsSparse;
+    R.scan_blocks = (R.total_cells + R.scan_cells - 1) / R.scan_cells;
     R.list_off = list_off;
     R.prof_off = (uint32_t)prof_off;
     if (R.prof_bins) prof_off += (uint64_t)R.prof_bins * kProfileSums + 3u;
@@ -478,10 +480,10 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     }
     for (uint32_t b = 0; b < R.count; b += kBlock)
       s->atom_blocks.push_back({(uint32_t)r, atom_off + b, atom_off + std::min(R.count, b + kBlock), 0});
-    for (uint32_t b = 0; b < R.total_cells; b += kScanCells)
+    for (uint32_t b = 0; b < R.total_cells; b += R.scan_cells)
       s->scan_blocks.push_back(
-          {(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + kScanCells), 0});
-    const uint32_t per = (double)R.count >= kSortDense * (double)R.total_cells ? kSortCells : kSortCellsSparse;
+          {(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + R.scan_cells), 0});
+    const uint32_t per = dense ? kSortCells : kSortCellsSparse;
     for (uint32_t b = 0; b < R.total_cells; b += per)
       s->sort_blocks.push_back({(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + per), 0});
     atom_off += R.count;
