@@ -279,10 +279,10 @@ int vlj_debug_lists(vlj_sim *sim, uint32_t *counts, uint32_t *entries);
 int vlj_debug_grid(const vlj_sim *sim, uint32_t out[6]);
 /* Totals over the handle's replicas: list rebuilds, brick list builds that
  * cut the brick into parts (its neighbourhood outgrew LDS), the largest list
- * overflow, the number of bricks, and list builds of single home cells that
- * read global memory (not even one cell's neighbourhood fits). No
- * counterpart in the reference. */
-int vlj_debug_counters(const vlj_sim *sim, uint64_t out[5]);
+ * overflow, the number of bricks, list builds of single home cells that
+ * read global memory (not even one cell's neighbourhood fits), and the
+ * number of bricks with the sparse layout. No counterpart in the reference. */
+int vlj_debug_counters(const vlj_sim *sim, uint64_t out[6]);
 /* Derived reals of replica 0: energy_scale, momentum_scale, shift,
  * force_shift, c1, c2. */
 int vlj_debug_params(const vlj_sim *sim, double out[6]);

@@ -166,11 +166,14 @@ def test_dense_cluster_takes_the_global_memory_path(oracle, lj):
     assert sim.debug_counters()["global_cell_builds"] > 0
 
 
-def test_bricks_cut_into_parts_stay_exact(oracle, lj, monkeypatch):
+@pytest.mark.parametrize("layout", ["dense", "sparse"])
+def test_bricks_cut_into_parts_stay_exact(oracle, lj, monkeypatch, layout):
     # bricks planned far beyond what LDS holds (8 x 8 home cells of a
-    # liquid stage ~5700 atoms): each is cut into runs of home rows; lists,
+    # liquid stage ~5700 atoms): each is cut into runs of home rows (dense
+    # layout, 3200 atoms) or of home cells (sparse layout, 1024); lists,
     # forces, measurements, profiles and the heat current stay exact
     monkeypatch.setenv("VLJ_STAGE_FILL", "100")
+    monkeypatch.setenv("VLJ_SPARSE_FILL", "100" if layout == "sparse" else "0")
     size = [24.0, 24.0, 12.0]
     count = int(round(0.8442 * np.prod(size)))
     pos, vel = oracle.lattice_box(count, size, 1.0, 5)
@@ -196,6 +199,24 @@ def test_bricks_cut_into_parts_stay_exact(oracle, lj, monkeypatch):
     c = sim.debug_counters()
     assert c["bricks"] == 4 and c["stage_overflow_builds"] >= 4 * c["rebuilds"]
     assert c["global_cell_builds"] == 0
+    assert c["sparse_bricks"] == (4 if layout == "sparse" else 0)
+
+
+def test_gas_on_sparse_bricks_matches_the_oracle(oracle, lj):
+    # a gas at rho 0.05: its layers take the sparse layout (128-thread
+    # workgroups staging up to 1024 atoms); Langevin trajectory bit-exact
+    side = float(np.float32((3000 / 0.05) ** (1.0 / 3.0)))
+    parts = lj.random_insertion_array(3000, [side] * 3, 0.9, 1.5, 17)
+    pos, vel = np.ascontiguousarray(parts["pos"]), np.ascontiguousarray(parts["vel"])
+    sim = lj.LennardJones((pos, vel), settings(lj, side, thermostat=lj.Langevin(1.5, 1.0, 23)))
+    ref = oracle.Sim(pos, vel, [side] * 3, dt=0.005, thermostat=1, temperature=1.5, friction=1.0, seed=23)
+    c = sim.debug_counters()
+    assert c["sparse_bricks"] == c["bricks"] > 0
+    for steps in (1, 150):
+        sim.run(steps)
+        ref.run(steps)
+        assert list(sim.measure_now().raw) == list(ref.measure().raw)
+        np.testing.assert_array_equal(sim.debug_state()[0], ref.state()[0])
 
 
 def test_two_cells_per_axis(oracle, lj):

@@ -38,10 +38,12 @@ constexpr int kStageCells = 3 * (kBrickMax + 2) * (kBrickMax + 2);  // staged ce
 constexpr int kHomeCells = kBrickMax * kBrickMax;                    // home cells, at most
 static_assert(kHomeCells <= 64, "one lane of a wave per home cell");
 constexpr uint32_t kPartsGrid = 128;  // workgroups of a launch over the parts of cut bricks
-constexpr uint32_t kCutGroup = VLJ_BRICK_THREADS;  // bricks per group with a count of cut bricks
-static_assert(kStageCells <= 2 * kBrickThreads, "staging takes two cells per thread");
+constexpr uint32_t kCutGroup = 128;   // bricks per group with a count of cut bricks
 #ifndef VLJ_STAGE_MAX
 #define VLJ_STAGE_MAX 3200
+#endif
+#ifndef VLJ_SPARSE_STAGE_MAX
+#define VLJ_SPARSE_STAGE_MAX 1024
 #endif
 // Atoms a brick workgroup stages in LDS. With the sentinel the staged arrays
 // hold kStageMax + 64 doubles each (a multiple of 64, for ds_read2st64); at
@@ -51,6 +53,26 @@ static_assert(kStageCells <= 2 * kBrickThreads, "staging takes two cells per thr
 // step; 512 / 3008: 0.338; 512 / 3200 with bricks planned to 90%: 0.332.
 constexpr int kStageMax = VLJ_STAGE_MAX;
 static_assert(kStageMax % 64 == 0, "staged arrays are whole rows of 64");
+
+// A brick layout: threads of its workgroup and atoms it stages. Dense bricks
+// (liquid layers) use 512 threads and 3200 atoms, two workgroups per CU.
+// Sparse bricks, whose neighbourhood holds at most ~900 atoms (a gas: an
+// 8 x 8 brick of a replica at rho 0.1 stages ~650 and owns ~140), use 128
+// threads and 1024 atoms: five workgroups per CU instead of two, each with
+// most of its lanes busy. The brick table holds the dense bricks first, then
+// the sparse ones; each layout has its own launches.
+template <int THREADS, int STAGE_MAX, bool SPARSE>
+struct Layout {
+  static constexpr int kThreads = THREADS;
+  static constexpr int kStageMax = STAGE_MAX;
+  static constexpr uint32_t kListPad = (uint32_t)STAGE_MAX;  // a list pad: the sentinel's staged index
+  static constexpr bool kSparse = SPARSE;
+  static constexpr int kCellsPerThread = (kStageCells + THREADS - 1) / THREADS;  // staged cells per thread
+  static_assert(THREADS % 64 == 0 && THREADS <= 1024 && THREADS >= (int)kCutGroup, "whole waves");
+  static_assert(STAGE_MAX % 64 == 0 && STAGE_MAX < 65535, "whole rows of 64; 16-bit staged indices");
+};
+using Dense = Layout<kBrickThreads, kStageMax, false>;
+using Sparse = Layout<128, VLJ_SPARSE_STAGE_MAX, true>;
 constexpr int kListStride = 64;   // lists are transposed per block of 64 atoms (one wave)
 constexpr int kSlots = 8;         // fixed-point slots per replica
 constexpr double kNearR2 = 2.25;  // condensed: neighbours within 1.5 sigma (lennard_jones.wgsl:417)
@@ -121,11 +143,11 @@ struct Buffers {
   const Range* atom_blocks;
   const Range* scan_blocks;
   const Range* sort_blocks;
-  const Brick* bricks;        // list build, forces and measurements
+  const Brick* bricks;        // list build, forces and measurements: n_dense dense, then sparse
   uint32_t* cut;              // per brick: parts it was cut into at its replica's last list build
   uint16_t* parts;            // per brick: kHomeCells packed parts
   uint32_t* cut_groups;       // per group of kCutGroup bricks: bricks cut
-  uint32_t n_atom_blocks, n_scan_blocks, n_sort_blocks, n_bricks, n_replicas;
+  uint32_t n_atom_blocks, n_scan_blocks, n_sort_blocks, n_bricks, n_replicas, n_dense;
   unsigned long long* bmax;   // per atom block: max moved^2 (FP64 bits)
   long long* kepart;          // per atom block: kinetic energy in fixed point (Nose-Hoover)
   long long* hexpart;         // per atom block: kHexSums heat-exchange sums
@@ -178,11 +200,10 @@ __host__ __device__ __forceinline__ uint32_t dealt_index(uint32_t b) {
 //   list_off + (li / 64) * octets * 512 + o * 512 + (li % 64) * 8,
 // octets = ceil(capacity / 8). A wave reads octet o of its 64 atoms as 1 KB
 // of contiguous memory; all of a wave's lists lie in octets KB. The last
-// octet of a list is filled up with kListPad, the staged index of a sentinel
+// octet of a list is filled up with the layout's kListPad, the staged index of a sentinel
 // position far outside every box, so the force loop can take whole octets
 // without a per-entry bound.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-constexpr uint32_t kListPad = (uint32_t)kStageMax;
 constexpr double kFar = 1.0e18;  // sentinel coordinate: r^2 ~ 3e36, beyond any cutoff
 
 __host__ __device__ __forceinline__ uint32_t list_octets(uint32_t capacity) { return (capacity + 7u) / 8u; }

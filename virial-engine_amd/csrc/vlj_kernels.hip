@@ -592,6 +592,7 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Buffers B) {
 // added, so a displacement is one subtraction (R4, R5). Every rebuild puts
 // the atoms in cell order (R8), so staged cell c holds the slots
 // start[c] .. start[c] + (off[c + 1] - off[c]).
+template <class L>
 struct Stage {
   uint32_t start[kStageCells];
   uint32_t off[kStageCells + 1];
@@ -599,9 +600,9 @@ struct Stage {
   uint8_t image[kStageCells];     // (ix + 1) + 3 (iy + 1) + 9 (iz + 1), i in {-1, 0, 1}
   uint32_t ns;                    // staged cells
   uint32_t total;                 // staged atoms
-  // Staged positions and, at kListPad, the sentinel. The arrays are 2112
-  // long (33 x 64 doubles) so that x[e] and y[e] are one ds_read2st64_b64.
-  double x[kStageMax + 64], y[kStageMax + 64], z[kStageMax + 64];
+  // Staged positions and, at L::kListPad, the sentinel. The arrays are a
+  // multiple of 64 doubles long so that x[e] and y[e] are one ds_read2st64_b64.
+  double x[L::kStageMax + 64], y[L::kStageMax + 64], z[L::kStageMax + 64];
 };
 
 // Staged index of the cell at offset (dz, yy, xx) from the brick's corner
@@ -611,7 +612,8 @@ __device__ __forceinline__ uint32_t staged_cell(const Brick& g, uint32_t dz, uin
 }
 
 // Staged cell of staged atom e (binary search over the staged offsets).
-__device__ __forceinline__ uint32_t stage_cell_of(const Stage& S, uint32_t e) {
+template <class L>
+__device__ __forceinline__ uint32_t stage_cell_of(const Stage<L>& S, uint32_t e) {
   uint32_t lo = 0, hi = S.ns - 1u;
   while (lo < hi) {
     const uint32_t mid = (lo + hi + 1u) >> 1;
@@ -628,20 +630,23 @@ __device__ __forceinline__ void image_shift(const Replica& R, uint32_t code, dou
 
 // Fills S for brick g from the positions X, Y, Z. Every thread of the
 // workgroup calls it. Returns whether the positions were staged (the
-// neighbourhood holds at most kStageMax atoms).
+// neighbourhood holds at most L::kStageMax atoms).
+template <class L>
 __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, const double* X, const double* Y,
-                            const double* Z, Stage& S) {
+                            const double* Z, Stage<L>& S) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
   const int c0 = (int)g.cell0;
   const int cx0 = c0 % nx, cy0 = (c0 / nx) % ny, cz = c0 / (nx * ny);
   const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
-  const uint32_t ns = 3u * sy * sx;  // at most kStageCells <= 2 kBrickThreads
+  const uint32_t ns = 3u * sy * sx;  // at most kStageCells
   const uint32_t t = threadIdx.x;
-  // thread t stages cells 2t and 2t + 1
-  uint32_t n_c[2] = {0u, 0u};
+  // thread t stages the kCellsPerThread consecutive cells from t * kCellsPerThread
+  constexpr uint32_t kCPT = (uint32_t)L::kCellsPerThread;
+  uint32_t n_c[kCPT];
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const uint32_t c = 2u * t + (uint32_t)h;
+  for (uint32_t h = 0; h < kCPT; h++) {
+    const uint32_t c = kCPT * t + h;
+    n_c[h] = 0u;
     if (c < ns) {
       const int xx = (int)(c % sx), yy = (int)((c / sx) % sy), dz = (int)(c / (sx * sy));
       const int ux = cx0 - 1 + xx, uy = cy0 - 1 + yy, uz = cz - 1 + dz;  // unwrapped
@@ -657,18 +662,23 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
     }
   }
   // exclusive scan of the staged counts over the workgroup
-  __shared__ uint32_t wtot[kBrickThreads / 64];
-  const uint32_t n_t = n_c[0] + n_c[1];
+  __shared__ uint32_t wtot[L::kThreads / 64];
+  uint32_t n_t = 0;
+#pragma unroll
+  for (uint32_t h = 0; h < kCPT; h++) n_t += n_c[h];
   const int lane = t & 63, wave = t >> 6;
   const uint32_t incl = wave_incl_scan_u32(n_t, lane);
   if (lane == 63) wtot[wave] = incl;
   __syncthreads();
   uint32_t before = 0;
   for (int w = 0; w < wave; w++) before += wtot[w];
-  const uint32_t first = before + incl - n_t;
-  if (2u * t < ns) S.off[2u * t] = first;
-  if (2u * t + 1u < ns) S.off[2u * t + 1u] = first + n_c[0];
-  if (t == kBrickThreads - 1) {
+  uint32_t run = before + incl - n_t;
+#pragma unroll
+  for (uint32_t h = 0; h < kCPT; h++) {
+    if (kCPT * t + h < ns) S.off[kCPT * t + h] = run;
+    run += n_c[h];
+  }
+  if (t == (uint32_t)L::kThreads - 1u) {
     S.off[ns] = before + incl;
     S.total = before + incl;
     S.ns = ns;
@@ -686,15 +696,15 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
     if (t + 1u == nh) S.hoff[nh] = incl;
   }
   const uint32_t total = S.total;
-  const bool staged = total <= (uint32_t)kStageMax;
+  const bool staged = total <= (uint32_t)L::kStageMax;
   if (staged) {
     // every load of this thread's entries first, then the LDS stores
-    constexpr int kPer = (kStageMax + kBrickThreads - 1) / kBrickThreads;
+    constexpr int kPer = (L::kStageMax + L::kThreads - 1) / L::kThreads;
     uint32_t code[kPer];
     double px[kPer], py[kPer], pz[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
-      const uint32_t e = t + (uint32_t)k * kBrickThreads;
+      const uint32_t e = t + (uint32_t)k * L::kThreads;
       code[k] = 13u;
       px[k] = py[k] = pz[k] = 0.0;
       if (e < total) {
@@ -708,7 +718,7 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
     }
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
-      const uint32_t e = t + (uint32_t)k * kBrickThreads;
+      const uint32_t e = t + (uint32_t)k * L::kThreads;
       if (e < total) {
         double shx, shy, shz;
         image_shift(R, code[k], shx, shy, shz);
@@ -717,7 +727,7 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
         S.z[e] = pz[k] + shz;
       }
     }
-    if (t == 0) S.x[kListPad] = S.y[kListPad] = S.z[kListPad] = kFar;
+    if (t == 0) S.x[L::kListPad] = S.y[L::kListPad] = S.z[L::kListPad] = kFar;
   }
   __syncthreads();
   return staged;
@@ -728,7 +738,8 @@ struct Home {
   uint32_t hx, hy, e, slot;
 };
 
-__device__ __forceinline__ Home home_atom(const Brick& g, const Stage& S, uint32_t h) {
+template <class L>
+__device__ __forceinline__ Home home_atom(const Brick& g, const Stage<L>& S, uint32_t h) {
   // the last home cell whose first home atom is at most h (binary search)
   uint32_t hc = 0, hi = g.gx * g.gy - 1u;
   while (hc < hi) {
@@ -747,7 +758,8 @@ __device__ __forceinline__ Home home_atom(const Brick& g, const Stage& S, uint32
 
 // Position of staged atom e, with its image: from LDS when staged, else
 // looked up through the cell tables in global memory.
-__device__ __forceinline__ void staged_pos(const Replica& R, const Stage& S, bool staged, uint32_t e,
+template <class L>
+__device__ __forceinline__ void staged_pos(const Replica& R, const Stage<L>& S, bool staged, uint32_t e,
                                            const double* X, const double* Y, const double* Z, double& x,
                                            double& y, double& z) {
   if (staged) {
@@ -777,14 +789,17 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__bui
 // same way, from the same cell counts, so the staged indices of a list refer
 // to the staging of its part. A part is packed as x0 | y0 << 4 | w << 8 |
 // h << 12 (home-cell offsets and sizes within the brick).
-__device__ __forceinline__ uint32_t part_rows(const Brick& g, const Stage& S, uint32_t y0, uint32_t h) {
+template <class L>
+__device__ __forceinline__ uint32_t part_rows(const Brick& g, const Stage<L>& S, uint32_t y0, uint32_t h) {
   const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
   uint32_t n = 0;
   for (uint32_t dz = 0; dz < 3u; dz++) n += S.off[(dz * sy + y0 + h + 2u) * sx] - S.off[(dz * sy + y0) * sx];
   return n;
 }
 
-__device__ __forceinline__ uint32_t part_cells(const Brick& g, const Stage& S, uint32_t y0, uint32_t x0, uint32_t w) {
+template <class L>
+__device__ __forceinline__ uint32_t part_cells(const Brick& g, const Stage<L>& S, uint32_t y0, uint32_t x0,
+                                               uint32_t w) {
   const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
   uint32_t n = 0;
   for (uint32_t dz = 0; dz < 3u; dz++)
@@ -796,8 +811,9 @@ __device__ __forceinline__ uint32_t part_cells(const Brick& g, const Stage& S, u
 }
 
 // The parts of brick g from its staged cell offsets (one thread).
-__device__ uint32_t cut_brick(const Brick& g, const Stage& S, uint16_t* parts) {
-  const uint32_t lim = (uint32_t)kStageMax;
+template <class L>
+__device__ uint32_t cut_brick(const Brick& g, const Stage<L>& S, uint16_t* parts) {
+  const uint32_t lim = (uint32_t)L::kStageMax;
   uint32_t np = 0;
   for (uint32_t y0 = 0; y0 < g.gy;) {
     uint32_t h = 1;
@@ -821,17 +837,18 @@ __device__ __forceinline__ Brick part_brick(const Brick& g, uint32_t c, const Re
   return Brick{g.rep, g.cell0 + ((c >> 4) & 15u) * R.cells[0] + (c & 15u), (c >> 8) & 15u, c >> 12};
 }
 
-// Workgroup-wide exclusive scan of one value per thread (kBrickThreads
-// threads); *total gets the sum. Every thread calls it.
+// Workgroup-wide exclusive scan of one value per thread (THREADS threads);
+// *total gets the sum. Every thread calls it.
+template <int THREADS>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
-  __shared__ uint32_t wsum[kBrickThreads / 64];
+  __shared__ uint32_t wsum[THREADS / 64];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t incl = wave_incl_scan_u32(v, (int)lane);
   __syncthreads();  // earlier readers of wsum are done
   if (lane == 63u) wsum[wave] = incl;
   __syncthreads();
   uint32_t before = 0, all = 0;
-  for (uint32_t w = 0; w < (uint32_t)(kBrickThreads / 64); w++) {
+  for (uint32_t w = 0; w < (uint32_t)(THREADS / 64); w++) {
     const uint32_t x = wsum[w];
     before += w < wave ? x : 0u;
     all += x;
@@ -847,29 +864,41 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total)
 // of one brick run side by side. Every workgroup finds the cut bricks the
 // same way: the table comes in groups of kCutGroup bricks, each group with
 // a count of its cut bricks, and only groups that hold some are read.
-template <class F>
+// The range of the brick table that layout L covers: the dense bricks
+// first, then the sparse ones.
+template <class L>
+__device__ __forceinline__ uint32_t layout_first(const Buffers& B) {
+  return L::kSparse ? B.n_dense : 0u;
+}
+template <class L>
+__device__ __forceinline__ uint32_t layout_end(const Buffers& B) {
+  return L::kSparse ? B.n_bricks : B.n_dense;
+}
+
+template <class L, class F>
 __device__ __forceinline__ void each_cut_part(const Buffers& B, F&& f) {
-  static_assert(kCutGroup == (uint32_t)kBrickThreads, "a thread per brick of a group");
-  __shared__ uint32_t glist[kBrickThreads];  // groups with cut bricks, of this round
-  __shared__ uint32_t blist[kBrickThreads];  // cut bricks of the current group
-  __shared__ uint32_t first[kBrickThreads];  // number of each one's first part within the group
+  __shared__ uint32_t glist[L::kThreads];  // groups with cut bricks, of this round
+  __shared__ uint32_t blist[kCutGroup];    // cut bricks of the current group
+  __shared__ uint32_t first[kCutGroup];    // number of each one's first part within the group
   if (B.flags[1] == 0u) return;  // no brick is cut
   const uint32_t t = threadIdx.x, G = gridDim.x;
+  const uint32_t b0 = layout_first<L>(B), b1 = layout_end<L>(B);
   const uint32_t groups = (B.n_bricks + kCutGroup - 1u) / kCutGroup;
   uint32_t base = 0;  // parts of the groups before
-  for (uint32_t g0 = 0; g0 < groups; g0 += kBrickThreads) {
+  for (uint32_t g0 = 0; g0 < groups; g0 += (uint32_t)L::kThreads) {
     const uint32_t grp = g0 + t;
     const uint32_t has = grp < groups && B.cut_groups[grp] != 0u ? 1u : 0u;
     uint32_t n_groups;
-    const uint32_t gpos = block_excl_scan(has, &n_groups);  // past every earlier use of the lists
+    const uint32_t gpos = block_excl_scan<L::kThreads>(has, &n_groups);  // past every earlier use of the lists
     if (has) glist[gpos] = grp;
     __syncthreads();
     for (uint32_t k = 0; k < n_groups; k++) {
+      // a thread per brick of the group; bricks of the other layout count as uncut
       const uint32_t gb = uniform(glist[k]) * kCutGroup + t;
-      const uint32_t np = gb < B.n_bricks ? B.cut[gb] : 0u;
+      const uint32_t np = t < kCutGroup && gb >= b0 && gb < b1 ? B.cut[gb] : 0u;
       uint32_t n_cut, n_parts;
-      const uint32_t bpos = block_excl_scan(np != 0u ? 1u : 0u, &n_cut);
-      const uint32_t upos = block_excl_scan(np, &n_parts);
+      const uint32_t bpos = block_excl_scan<L::kThreads>(np != 0u ? 1u : 0u, &n_cut);
+      const uint32_t upos = block_excl_scan<L::kThreads>(np, &n_parts);
       if (np != 0u) {
         blist[bpos] = gb;
         first[bpos] = upos;
@@ -934,7 +963,8 @@ static_assert(kListStride == 64, "entry pos of a list lies (pos / 8) KB + 2 (pos
 // On a reorder the workgroup also copies its home atoms' state back from
 // the second set.
 // The lists of the home atoms of brick (or part) g, staged in S.
-__device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+template <class L>
+__device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                            bool staged, bool moved, const double* X, const double* Y,
                                            const double* Z, uint32_t& next_item) {
   const uint32_t cap = R.capacity;
@@ -942,7 +972,7 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
   if (!staged && threadIdx.x == 0) atomicAdd(&B.ctl[g.rep].stage_global, 1u);
   const uint32_t nh = g.gx * g.gy;
   const uint32_t H = S.hoff[nh];
-  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
+  for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home a = home_atom(g, S, h);
     const uint32_t i = a.slot;
     double xi, yi, zi;
@@ -1022,14 +1052,14 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
 #pragma unroll
       for (int c = 0; c < kGroup; c++) {
         const uint32_t a = c0 + 64u * (uint32_t)c, s = a + lane;
-        q[c] = kListPad;
+        q[c] = L::kListPad;
 #pragma unroll
         for (int r = 8; r >= 0; r--)  // the first run that ends past s
           if (rend[r] > a && (r == 0 || rend[r - 1] < a + 64u) && s < rend[r]) q[c] = s + rdelta[r];
         const bool valid = s < T;
         cx[c] = cy[c] = cz[c] = kFar;
         if (staged) {
-          const uint32_t qs = valid ? q[c] : kListPad;
+          const uint32_t qs = valid ? q[c] : L::kListPad;
           cx[c] = S.x[qs];
           cy[c] = S.y[qs];
           cz[c] = S.z[qs];
@@ -1074,7 +1104,7 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
       const uint32_t li = lbase + k0 + lane;
       const uint32_t len = found < cap ? found : cap;
       uint16_t* lst = B.nbr + list_base(R, li);
-      for (uint32_t p = len; (p & 7u) != 0u; p++) lst[list_entry(p)] = (uint16_t)kListPad;
+      for (uint32_t p = len; (p & 7u) != 0u; p++) lst[list_entry(p)] = (uint16_t)L::kListPad;
       B.nbr_count[R.atom_off + li] = len;
       if (found > cap) atomicMax(&B.ctl[g.rep].overflow, found);
     }
@@ -1085,14 +1115,14 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
 // marked to be cut and builds them part by part (PARTS true; launched after
 // the first, on the same step). A brick counts as cut when its
 // neighbourhood does not fit in LDS and it has more than one home cell.
-template <bool PARTS>
-__global__ void __launch_bounds__(kBrickThreads) k_build(Buffers B) {
-  __shared__ Stage S;
+template <class L, bool PARTS>
+__global__ void __launch_bounds__(L::kThreads) k_build(Buffers B) {
+  __shared__ Stage<L> S;
   __shared__ uint32_t next_item;
   if (B.flags[0] == 0u) return;  // no replica rebuilds this step
   if constexpr (!PARTS) {
-    const uint32_t gb = dealt_index(blockIdx.x);
-    if (gb >= B.n_bricks) return;
+    const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
+    if (gb >= layout_end<L>(B)) return;
     const Brick g = B.bricks[gb];
     const Control& ctl = B.ctl[g.rep];
     if (!ctl.rebuild) return;
@@ -1117,7 +1147,7 @@ __global__ void __launch_bounds__(kBrickThreads) k_build(Buffers B) {
     if (cut) return;
     build_part(B, R, S, g, staged, moved, X, Y, Z, next_item);
   } else {
-    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+    each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
       const Brick brick = B.bricks[gb];
       const Control& ctl = B.ctl[brick.rep];
       if (!ctl.rebuild) return;
@@ -1239,8 +1269,8 @@ __device__ __forceinline__ u32x4 load_octet(const uint16_t* base, uint32_t o, ui
 // come from the brick's LDS; whole octets are taken (pads meet the far
 // sentinel), so the eight pairs of an octet carry no branch and their LDS
 // reads can all be in flight together.
-template <int PREC, bool SF, bool ENERGIES>
-__device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const Replica& R, const Stage& S,
+template <int PREC, bool SF, bool ENERGIES, class L>
+__device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const Replica& R, const Stage<L>& S,
                                                        uint32_t i, double xi, double yi, double zi) {
   const uint32_t cnt = B.nbr_count[i];
   const uint32_t no = (cnt + 7u) / 8u;
@@ -1277,8 +1307,8 @@ __device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const R
 
 // The same sum for a brick whose neighbourhood did not fit in LDS: entry by
 // entry up to the count, positions through the cell tables.
-template <int PREC, bool SF, bool ENERGIES>
-__device__ __forceinline__ PairAcc interactions_global(const Buffers& B, const Replica& R, const Stage& S,
+template <int PREC, bool SF, bool ENERGIES, class L>
+__device__ __forceinline__ PairAcc interactions_global(const Buffers& B, const Replica& R, const Stage<L>& S,
                                                     uint32_t i, double xi, double yi, double zi) {
   const uint32_t cnt = B.nbr_count[i];
   const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
@@ -1292,8 +1322,8 @@ __device__ __forceinline__ PairAcc interactions_global(const Buffers& B, const R
   return a;
 }
 
-template <int PREC, bool SF, bool ENERGIES>
-__device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica& R, const Stage& S, bool staged,
+template <int PREC, bool SF, bool ENERGIES, class L>
+__device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica& R, const Stage<L>& S, bool staged,
                                                 uint32_t i, double xi, double yi, double zi) {
   if (staged) return interactions_staged<PREC, SF, ENERGIES>(B, R, S, i, xi, yi, zi);
   return interactions_global<PREC, SF, ENERGIES>(B, R, S, i, xi, yi, zi);
@@ -1301,11 +1331,11 @@ __device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica&
 
 // New accelerations and (kick) the second half kick v += a dt/2, one
 // workgroup per brick.
-template <int PREC, bool SF>
-__device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+template <int PREC, bool SF, class L>
+__device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                            bool staged, int kick) {
   const uint32_t H = S.hoff[g.gx * g.gy];
-  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
+  for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
     double xi, yi, zi;
@@ -1325,19 +1355,19 @@ __device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, c
 
 // One workgroup per brick, leaving bricks that are cut (see k_build) to the
 // PARTS launch that follows.
-template <int PREC, bool SF, bool PARTS>
-__global__ void __launch_bounds__(kBrickThreads) k_force(Buffers B, int kick) {
-  __shared__ Stage S;
+template <class L, int PREC, bool SF, bool PARTS>
+__global__ void __launch_bounds__(L::kThreads) k_force(Buffers B, int kick) {
+  __shared__ Stage<L> S;
   if constexpr (!PARTS) {
-    const uint32_t gb = dealt_index(blockIdx.x);
-    if (gb >= B.n_bricks) return;
+    const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
+    if (gb >= layout_end<L>(B)) return;
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
     if (!staged && g.gx * g.gy > 1u) return;
     force_part<PREC, SF>(B, R, S, g, staged, kick);
   } else {
-    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+    each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
       const Brick brick = B.bricks[gb];
       const Replica R = B.rep[brick.rep];
       const Brick g = part_brick(brick, code, R);
@@ -1375,13 +1405,13 @@ __device__ __forceinline__ void block_add_slots(long long v[kSlots], long long* 
   }
 }
 
-template <int PREC, bool SF>
-// Each pass over up to kBrickThreads home atoms ends in the workgroup's sum
+// Each pass over up to L::kThreads home atoms ends in the workgroup's sum
 // (a pass's sums are not kept across the pair loop).
-__device__ __forceinline__ void measure_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+template <int PREC, bool SF, class L>
+__device__ __forceinline__ void measure_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                              bool staged, long long* dst) {
   const uint32_t H = S.hoff[g.gx * g.gy];
-  for (uint32_t h0 = 0; h0 < H; h0 += kBrickThreads) {
+  for (uint32_t h0 = 0; h0 < H; h0 += L::kThreads) {
     long long v[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
     const uint32_t h = h0 + threadIdx.x;
     if (h < H) {
@@ -1401,24 +1431,24 @@ __device__ __forceinline__ void measure_part(const Buffers& B, const Replica& R,
       v[6] = a.near >= R.near_count ? 1 : 0;
       v[7] = (long long)a.within;
     }
-    block_add_slots<kSlots, kBrickThreads>(v, dst);
+    block_add_slots<kSlots, L::kThreads>(v, dst);
     __syncthreads();  // the sum's LDS is free for the next pass
   }
 }
 
-template <int PREC, bool SF, bool PARTS>
-__global__ void __launch_bounds__(kBrickThreads) k_measure(Buffers B) {
-  __shared__ Stage S;
+template <class L, int PREC, bool SF, bool PARTS>
+__global__ void __launch_bounds__(L::kThreads) k_measure(Buffers B) {
+  __shared__ Stage<L> S;
   if constexpr (!PARTS) {
-    const uint32_t gb = dealt_index(blockIdx.x);
-    if (gb >= B.n_bricks) return;
+    const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
+    if (gb >= layout_end<L>(B)) return;
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
     if (!staged && g.gx * g.gy > 1u) return;
     measure_part<PREC, SF>(B, R, S, g, staged, B.slots + (size_t)g.rep * kSlots);
   } else {
-    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+    each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
       const Brick brick = B.bricks[gb];
       const Replica R = B.rep[brick.rep];
       const Brick g = part_brick(brick, code, R);
@@ -1462,14 +1492,14 @@ struct ProfileSink {
   __device__ __forceinline__ void add(uint32_t k, double x) const { add(k, to_fixed(x, kProfileOne)); }
 };
 
-template <bool SF>
-__device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+template <bool SF, class L>
+__device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                              bool staged, const ProfileSink& sink, long long whole[3]) {
   const uint32_t bins = R.prof_bins;
   const double width = R.bin_width;
   const int ibins = (int)bins;
   const uint32_t H = S.hoff[g.gx * g.gy];
-  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
+  for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
     double xi, yi, zi;
@@ -1540,20 +1570,21 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
 }
 
 // Adds the workgroup's sums of one brick to the replica's profile.
+template <class L>
 __device__ __forceinline__ void profile_flush(const ProfileSink& sink, uint32_t bins, const long long whole[3]) {
   const uint32_t nsums = bins * kProfileSums + 3u;
 #pragma unroll
   for (int a = 0; a < 3; a++) sink.add(bins * kProfileSums + (uint32_t)a, whole[a]);
   if (sink.use_lds) {
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nsums; k += kBrickThreads)
+    for (uint32_t k = threadIdx.x; k < nsums; k += L::kThreads)
       if (sink.lds[k] != 0) atomicAdd((unsigned long long*)(sink.global + k), (unsigned long long)sink.lds[k]);
   }
 }
 
-template <bool SF, bool PARTS>
-__global__ void __launch_bounds__(kBrickThreads) k_profile(Buffers B) {
-  __shared__ Stage S;
+template <class L, bool SF, bool PARTS>
+__global__ void __launch_bounds__(L::kThreads) k_profile(Buffers B) {
+  __shared__ Stage<L> S;
   __shared__ long long acc[kProfileLdsBins * kProfileSums + 3];
   // brick gb whole (code 0) or one part of it
   auto profile = [&](uint32_t gb, uint32_t code) {
@@ -1566,18 +1597,18 @@ __global__ void __launch_bounds__(kBrickThreads) k_profile(Buffers B) {
     const Brick g = PARTS ? part_brick(brick, code, R) : brick;
     if (PARTS) __syncthreads();  // the previous part is done with S and acc
     if (sink.use_lds)
-      for (uint32_t k = threadIdx.x; k < nsums; k += kBrickThreads) acc[k] = 0;
+      for (uint32_t k = threadIdx.x; k < nsums; k += L::kThreads) acc[k] = 0;
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
     if (!PARTS && !staged && g.gx * g.gy > 1u) return;
     long long whole[3] = {0, 0, 0};
     profile_part<SF>(B, R, S, g, staged, sink, whole);
-    profile_flush(sink, bins, whole);
+    profile_flush<L>(sink, bins, whole);
   };
   if constexpr (!PARTS) {
-    const uint32_t gb = dealt_index(blockIdx.x);
-    if (gb < B.n_bricks) profile(gb, 0u);
+    const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
+    if (gb < layout_end<L>(B)) profile(gb, 0u);
   } else {
-    each_cut_part(B, profile);
+    each_cut_part<L>(B, profile);
   }
 }
 
@@ -1589,11 +1620,11 @@ __global__ void __launch_bounds__(kBrickThreads) k_profile(Buffers B) {
 // sample `sample` of the series. Pair terms in FP64.
 // ---------------------------------------------------------------------------
 
-template <bool SF>
-__device__ __forceinline__ void heat_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+template <bool SF, class L>
+__device__ __forceinline__ void heat_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                           bool staged, long long jsum[kSlots]) {
   const uint32_t H = S.hoff[g.gx * g.gy];
-  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
+  for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
     double xi, yi, zi;
@@ -1630,12 +1661,12 @@ __device__ __forceinline__ void heat_part(const Buffers& B, const Replica& R, co
   }
 }
 
-template <bool SF, bool PARTS>
-__global__ void __launch_bounds__(kBrickThreads) k_heat_current(Buffers B, uint32_t sample) {
-  __shared__ Stage S;
+template <class L, bool SF, bool PARTS>
+__global__ void __launch_bounds__(L::kThreads) k_heat_current(Buffers B, uint32_t sample) {
+  __shared__ Stage<L> S;
   if constexpr (!PARTS) {
-    const uint32_t gb = dealt_index(blockIdx.x);
-    if (gb >= B.n_bricks) return;
+    const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
+    if (gb >= layout_end<L>(B)) return;
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     if (R.hc_cap == 0u || sample >= R.hc_cap) return;
@@ -1643,9 +1674,9 @@ __global__ void __launch_bounds__(kBrickThreads) k_heat_current(Buffers B, uint3
     if (!staged && g.gx * g.gy > 1u) return;
     long long jsum[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
     heat_part<SF>(B, R, S, g, staged, jsum);
-    block_add_slots<3, kBrickThreads>(jsum, B.hc + 3ull * sample);
+    block_add_slots<3, L::kThreads>(jsum, B.hc + 3ull * sample);
   } else {
-    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+    each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
       const Brick brick = B.bricks[gb];
       const Replica R = B.rep[brick.rep];
       if (R.hc_cap == 0u || sample >= R.hc_cap) return;
@@ -1654,7 +1685,7 @@ __global__ void __launch_bounds__(kBrickThreads) k_heat_current(Buffers B, uint3
       const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
       long long jsum[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
       heat_part<SF>(B, R, S, g, staged, jsum);
-      block_add_slots<3, kBrickThreads>(jsum, B.hc + 3ull * sample);
+      block_add_slots<3, L::kThreads>(jsum, B.hc + 3ull * sample);
     });
   }
 }
@@ -1666,10 +1697,11 @@ __global__ void __launch_bounds__(kBrickThreads) k_heat_current(Buffers B, uint3
 // averaged over the molecule and its neighbours within the cutoff.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, const Stage& S, const Brick& g,
+template <class L>
+__device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                             bool staged) {
   const uint32_t H = S.hoff[g.gx * g.gy];
-  for (uint32_t h = threadIdx.x; h < H; h += kBrickThreads) {
+  for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
     double xi, yi, zi;
@@ -1703,19 +1735,19 @@ __device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, 
   }
 }
 
-template <bool PARTS>
-__global__ void __launch_bounds__(kBrickThreads) k_render(Buffers B) {
-  __shared__ Stage S;
+template <class L, bool PARTS>
+__global__ void __launch_bounds__(L::kThreads) k_render(Buffers B) {
+  __shared__ Stage<L> S;
   if constexpr (!PARTS) {
-    const uint32_t gb = dealt_index(blockIdx.x);
-    if (gb >= B.n_bricks) return;
+    const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
+    if (gb >= layout_end<L>(B)) return;
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
     if (!staged && g.gx * g.gy > 1u) return;
     render_part(B, R, S, g, staged);
   } else {
-    each_cut_part(B, [&](uint32_t gb, uint32_t code) {
+    each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
       const Brick brick = B.bricks[gb];
       const Replica R = B.rep[brick.rep];
       const Brick g = part_brick(brick, code, R);
@@ -1802,23 +1834,57 @@ hipError_t launch_heat_exchange(const Buffers& B, hipStream_t s) {
 // Grid of a launch over the bricks that are cut into parts.
 inline dim3 parts_grid(const Buffers& B) { return grid1(std::min<uint32_t>(B.n_bricks, kPartsGrid)); }
 
+// Brick kernels run as up to four launches: the dense bricks, the sparse
+// bricks, then the parts of cut bricks of each layout (which return after
+// one load when no brick is cut). Layouts without bricks are not launched.
+template <class L>
+inline uint32_t layout_count(const Buffers& B) {
+  return L::kSparse ? B.n_bricks - B.n_dense : B.n_dense;
+}
+template <class L>
+inline dim3 brick_grid(const Buffers& B) {
+  return grid1(dealt_grid(layout_count<L>(B)));
+}
+
+#define VLJ_BRICK_LAUNCHES(KERNEL, ...)                                                                   \
+  do {                                                                                                    \
+    if (layout_count<Dense>(B))                                                                           \
+      hipLaunchKernelGGL((KERNEL<Dense, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, __VA_ARGS__); \
+    if (layout_count<Sparse>(B))                                                                          \
+      hipLaunchKernelGGL((KERNEL<Sparse, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, __VA_ARGS__); \
+    if (layout_count<Dense>(B))                                                                           \
+      hipLaunchKernelGGL((KERNEL<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, __VA_ARGS__);  \
+    if (layout_count<Sparse>(B))                                                                          \
+      hipLaunchKernelGGL((KERNEL<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, __VA_ARGS__); \
+  } while (0)
+
 hipError_t launch_render(const Buffers& B, hipStream_t s) {
   if (!B.n_bricks || !B.render_pos) return hipSuccess;
-  hipLaunchKernelGGL(k_render<false>, grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
-  hipLaunchKernelGGL(k_render<true>, parts_grid(B), dim3(kBrickThreads), 0, s, B);
+  VLJ_BRICK_LAUNCHES(k_render, B);
   return hipGetLastError();
+}
+
+template <bool SF>
+void heat_launches(const Buffers& B, uint32_t sample, hipStream_t s) {
+  const dim3 gp = parts_grid(B);
+  if (layout_count<Dense>(B)) {
+    hipLaunchKernelGGL((k_heat_current<Dense, SF, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B, sample);
+  }
+  if (layout_count<Sparse>(B)) {
+    hipLaunchKernelGGL((k_heat_current<Sparse, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B,
+                       sample);
+  }
+  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_heat_current<Dense, SF, true>), gp, dim3(Dense::kThreads), 0, s, B, sample);
+  if (layout_count<Sparse>(B))
+    hipLaunchKernelGGL((k_heat_current<Sparse, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B, sample);
 }
 
 hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, hipStream_t s) {
   if (!B.n_bricks) return hipSuccess;
-  const dim3 g = grid1(dealt_grid(B.n_bricks)), t = dim3(kBrickThreads);
-  if (shifted) {
-    hipLaunchKernelGGL((k_heat_current<true, false>), g, t, 0, s, B, sample);
-    hipLaunchKernelGGL((k_heat_current<true, true>), parts_grid(B), t, 0, s, B, sample);
-  } else {
-    hipLaunchKernelGGL((k_heat_current<false, false>), g, t, 0, s, B, sample);
-    hipLaunchKernelGGL((k_heat_current<false, true>), parts_grid(B), t, 0, s, B, sample);
-  }
+  if (shifted)
+    heat_launches<true>(B, sample, s);
+  else
+    heat_launches<false>(B, sample, s);
   return hipGetLastError();
 }
 
@@ -1829,72 +1895,84 @@ hipError_t launch_rebuild(const Buffers& B, hipStream_t s) {
   hipLaunchKernelGGL(k_scatter, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_sort, grid1(dealt_grid(B.n_sort_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_reorder, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
-  if (B.n_bricks) {
-    hipLaunchKernelGGL(k_build<false>, grid1(dealt_grid(B.n_bricks)), dim3(kBrickThreads), 0, s, B);
-    hipLaunchKernelGGL(k_build<true>, parts_grid(B), dim3(kBrickThreads), 0, s, B);
-  }
+  if (B.n_bricks) VLJ_BRICK_LAUNCHES(k_build, B);
   return hipGetLastError();
+}
+
+template <int PREC, bool SF>
+void force_launches(const Buffers& B, int k, hipStream_t s) {
+  const dim3 gp = parts_grid(B);
+  if (layout_count<Dense>(B))
+    hipLaunchKernelGGL((k_force<Dense, PREC, SF, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B, k);
+  if (layout_count<Sparse>(B))
+    hipLaunchKernelGGL((k_force<Sparse, PREC, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B, k);
+  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_force<Dense, PREC, SF, true>), gp, dim3(Dense::kThreads), 0, s, B, k);
+  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_force<Sparse, PREC, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B, k);
 }
 
 hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s) {
   if (!B.n_atom_blocks) return hipSuccess;
-  const dim3 g = grid1(dealt_grid(B.n_bricks)), t = dim3(kBrickThreads), gp = parts_grid(B);
   const int k = kick ? 1 : 0;
   if (precision == 0) {
-    if (shifted) {
-      hipLaunchKernelGGL((k_force<0, true, false>), g, t, 0, s, B, k);
-      hipLaunchKernelGGL((k_force<0, true, true>), gp, t, 0, s, B, k);
-    } else {
-      hipLaunchKernelGGL((k_force<0, false, false>), g, t, 0, s, B, k);
-      hipLaunchKernelGGL((k_force<0, false, true>), gp, t, 0, s, B, k);
-    }
+    if (shifted)
+      force_launches<0, true>(B, k, s);
+    else
+      force_launches<0, false>(B, k, s);
   } else {
-    if (shifted) {
-      hipLaunchKernelGGL((k_force<1, true, false>), g, t, 0, s, B, k);
-      hipLaunchKernelGGL((k_force<1, true, true>), gp, t, 0, s, B, k);
-    } else {
-      hipLaunchKernelGGL((k_force<1, false, false>), g, t, 0, s, B, k);
-      hipLaunchKernelGGL((k_force<1, false, true>), gp, t, 0, s, B, k);
-    }
+    if (shifted)
+      force_launches<1, true>(B, k, s);
+    else
+      force_launches<1, false>(B, k, s);
   }
   return hipGetLastError();
+}
+
+template <int PREC, bool SF>
+void measure_launches(const Buffers& B, hipStream_t s) {
+  const dim3 gp = parts_grid(B);
+  if (layout_count<Dense>(B))
+    hipLaunchKernelGGL((k_measure<Dense, PREC, SF, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B);
+  if (layout_count<Sparse>(B))
+    hipLaunchKernelGGL((k_measure<Sparse, PREC, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B);
+  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_measure<Dense, PREC, SF, true>), gp, dim3(Dense::kThreads), 0, s, B);
+  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_measure<Sparse, PREC, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B);
 }
 
 hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStream_t s) {
   const uint32_t zb = (B.n_replicas * kSlots + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_zero_slots, grid1(zb), dim3(kBlock), 0, s, B);
   if (!B.n_atom_blocks) return hipGetLastError();
-  const dim3 g = grid1(dealt_grid(B.n_bricks)), t = dim3(kBrickThreads), gp = parts_grid(B);
   if (precision == 0) {
-    if (shifted) {
-      hipLaunchKernelGGL((k_measure<0, true, false>), g, t, 0, s, B);
-      hipLaunchKernelGGL((k_measure<0, true, true>), gp, t, 0, s, B);
-    } else {
-      hipLaunchKernelGGL((k_measure<0, false, false>), g, t, 0, s, B);
-      hipLaunchKernelGGL((k_measure<0, false, true>), gp, t, 0, s, B);
-    }
+    if (shifted)
+      measure_launches<0, true>(B, s);
+    else
+      measure_launches<0, false>(B, s);
   } else {
-    if (shifted) {
-      hipLaunchKernelGGL((k_measure<1, true, false>), g, t, 0, s, B);
-      hipLaunchKernelGGL((k_measure<1, true, true>), gp, t, 0, s, B);
-    } else {
-      hipLaunchKernelGGL((k_measure<1, false, false>), g, t, 0, s, B);
-      hipLaunchKernelGGL((k_measure<1, false, true>), gp, t, 0, s, B);
-    }
+    if (shifted)
+      measure_launches<1, true>(B, s);
+    else
+      measure_launches<1, false>(B, s);
   }
   return hipGetLastError();
 }
 
+template <bool SF>
+void profile_launches(const Buffers& B, hipStream_t s) {
+  const dim3 gp = parts_grid(B);
+  if (layout_count<Dense>(B))
+    hipLaunchKernelGGL((k_profile<Dense, SF, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B);
+  if (layout_count<Sparse>(B))
+    hipLaunchKernelGGL((k_profile<Sparse, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B);
+  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_profile<Dense, SF, true>), gp, dim3(Dense::kThreads), 0, s, B);
+  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_profile<Sparse, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B);
+}
+
 hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s) {
   if (!B.n_bricks) return hipSuccess;
-  const dim3 g = grid1(dealt_grid(B.n_bricks)), t = dim3(kBrickThreads);
-  if (shifted) {
-    hipLaunchKernelGGL((k_profile<true, false>), g, t, 0, s, B);
-    hipLaunchKernelGGL((k_profile<true, true>), parts_grid(B), t, 0, s, B);
-  } else {
-    hipLaunchKernelGGL((k_profile<false, false>), g, t, 0, s, B);
-    hipLaunchKernelGGL((k_profile<false, true>), parts_grid(B), t, 0, s, B);
-  }
+  if (shifted)
+    profile_launches<true>(B, s);
+  else
+    profile_launches<false>(B, s);
   return hipGetLastError();
 }
 

@@ -40,6 +40,7 @@ struct HostReplica {
   vlj_settings settings;
   double energy_scale, momentum_scale;
   std::vector<uint32_t> brick_x, brick_y;  // brick shape in cells, per z layer
+  std::vector<uint8_t> sparse;             // per z layer: its bricks use the sparse layout
   uint64_t prof_samples;      // profile samples since the last clear
 };
 
@@ -52,7 +53,8 @@ struct vlj_sim {
   std::vector<Replica> rep;
   std::vector<HostReplica> host;
   std::vector<Range> atom_blocks, scan_blocks, sort_blocks;
-  std::vector<Brick> bricks;
+  std::vector<Brick> bricks;  // n_dense dense bricks, then the sparse ones
+  uint32_t n_dense = 0;
   uint32_t n_total = 0, cells_total = 0;
   uint64_t list_total = 0;
   uint64_t prof_total = 0;  // profile sums over all replicas
@@ -343,11 +345,13 @@ int prime_if_needed(vlj_sim* s) {
 // and the replica's mean occupancy (a lattice that is not yet uniform evens
 // out). Window sums come from a 2D prefix sum of the three
 // layers' counts over the periodically extended plane. A brick that
-// outgrows LDS later on is cut into parts on the device. The environment
-// variable VLJ_STAGE_FILL, when set, replaces the 90% (a value above 1
-// plans bricks that will be cut: for tests and tuning).
+// outgrows LDS later on is cut into parts on the device. A layer whose
+// bricks are held to at most 90% of the sparse layout's 1024 atoms takes
+// that layout. The environment variables VLJ_STAGE_FILL and VLJ_SPARSE_FILL,
+// when set, replace the two 90% (a value above 1 plans bricks that will be
+// cut, 0 keeps every layer dense: for tests and tuning).
 void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vector<uint32_t>& bx,
-                   std::vector<uint32_t>& by) {
+                   std::vector<uint32_t>& by, std::vector<uint8_t>& sparse) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
 #ifndef VLJ_STAGE_FILL
 #define VLJ_STAGE_FILL 0.9
@@ -355,9 +359,11 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
 #ifndef VLJ_MEAN_FLOOR
 #define VLJ_MEAN_FLOOR 1.0
 #endif
-  double fill = VLJ_STAGE_FILL;
+  double fill = VLJ_STAGE_FILL, sparse_fill = VLJ_STAGE_FILL;
   if (const char* f = std::getenv("VLJ_STAGE_FILL")) fill = std::atof(f);
+  if (const char* f = std::getenv("VLJ_SPARSE_FILL")) sparse_fill = std::atof(f);
   const double limit = fill * kStageMax;
+  const double sparse_limit = sparse_fill * Sparse::kStageMax;
   const double mean = VLJ_MEAN_FLOOR * (double)R.count / (double)R.total_cells;
   const int ex = nx + 2, ey = ny + 2;  // plane extended by one cell each side
   const size_t plane = (size_t)(ex + 1) * (ey + 1);
@@ -386,8 +392,10 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
   };
   bx.assign(nz, 1);
   by.assign(nz, 1);
+  sparse.assign(nz, 0);
   for (int z = 0; z < nz; z++) {
     uint32_t best = 0;
+    double best_worst = limit;
     for (int gy = 1; gy <= std::min(kBrickMax, ny); gy++)
       for (int gx = gy; gx <= std::min(kBrickMax, nx); gx++) {
         const uint32_t home = (uint32_t)(gx * gy);
@@ -401,10 +409,13 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
         }
         if (worst <= limit) {
           best = home;
+          best_worst = worst;
           bx[z] = (uint32_t)gx;
           by[z] = (uint32_t)gy;
         }
       }
+    // a layer whose bricks stage few atoms takes the sparse layout
+    sparse[z] = best > 0 && best_worst <= sparse_limit ? 1 : 0;
   }
 }
 
@@ -470,14 +481,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     }
     R.atom_block_off = (uint32_t)s->atom_blocks.size();
     R.atom_blocks = (R.count + kBlock - 1) / kBlock;
-    choose_bricks(R, occ, s->host[r].brick_x, s->host[r].brick_y);
-    for (uint32_t z = 0; z < R.cells[2]; z++) {
-      const uint32_t gx = s->host[r].brick_x[z], gy = s->host[r].brick_y[z];
-      for (uint32_t y0 = 0; y0 < R.cells[1]; y0 += gy)
-          for (uint32_t x0 = 0; x0 < R.cells[0]; x0 += gx)
-            s->bricks.push_back({(uint32_t)r, (z * R.cells[1] + y0) * R.cells[0] + x0,
-                                 std::min(gx, R.cells[0] - x0), std::min(gy, R.cells[1] - y0)});
-    }
+    choose_bricks(R, occ, s->host[r].brick_x, s->host[r].brick_y, s->host[r].sparse);
     for (uint32_t b = 0; b < R.count; b += kBlock)
       s->atom_blocks.push_back({(uint32_t)r, atom_off + b, atom_off + std::min(R.count, b + kBlock), 0});
     for (uint32_t b = 0; b < R.total_cells; b += R.scan_cells)
@@ -492,6 +496,22 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     list_off += (uint64_t)list_octets(R.capacity) * 8u * ((R.count + kListStride - 1) / kListStride) * kListStride;
     if (R.thermostat == 1) s->any_langevin = true;
     if (R.thermostat == 2) s->any_nose_hoover = true;
+  }
+  // the brick table: every replica's dense bricks, then every sparse one
+  for (int pass = 0; pass < 2; pass++) {
+    for (size_t r = 0; r < replicas; r++) {
+      const Replica& R = s->rep[r];
+      const HostReplica& H = s->host[r];
+      for (uint32_t z = 0; z < R.cells[2]; z++) {
+        if ((H.sparse[z] != 0) != (pass == 1)) continue;
+        const uint32_t gx = H.brick_x[z], gy = H.brick_y[z];
+        for (uint32_t y0 = 0; y0 < R.cells[1]; y0 += gy)
+          for (uint32_t x0 = 0; x0 < R.cells[0]; x0 += gx)
+            s->bricks.push_back({(uint32_t)r, (z * R.cells[1] + y0) * R.cells[0] + x0,
+                                 std::min(gx, R.cells[0] - x0), std::min(gy, R.cells[1] - y0)});
+      }
+    }
+    if (pass == 0) s->n_dense = (uint32_t)s->bricks.size();
   }
   s->n_total = atom_off;
   s->cells_total = cell_off;
@@ -563,6 +583,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   B.sort_blocks = ob;
   B.bricks = gb;
   B.n_bricks = (uint32_t)s->bricks.size();
+  B.n_dense = s->n_dense;
   B.n_atom_blocks = (uint32_t)s->atom_blocks.size();
   B.n_scan_blocks = (uint32_t)s->scan_blocks.size();
   B.n_sort_blocks = (uint32_t)s->sort_blocks.size();
@@ -1206,7 +1227,7 @@ int vlj_debug_grid(const vlj_sim* sim, uint32_t out[6]) {
   return VLJ_OK;
 }
 
-int vlj_debug_counters(const vlj_sim* sim, uint64_t out[5]) {
+int vlj_debug_counters(const vlj_sim* sim, uint64_t out[6]) {
   if (!sim || !out) return VLJ_E_INVALID;
   std::vector<Control> c(sim->rep.size());
   if (!c.empty() && hipMemcpy(c.data(), sim->B.ctl, c.size() * sizeof(Control), hipMemcpyDeviceToHost) != hipSuccess)
@@ -1219,6 +1240,7 @@ int vlj_debug_counters(const vlj_sim* sim, uint64_t out[5]) {
     out[4] += x.stage_global;
   }
   out[3] = sim->bricks.size();
+  out[5] = sim->bricks.size() - sim->n_dense;
   return VLJ_OK;
 }
 
@@ -1538,7 +1560,7 @@ struct StateHeader {
   uint32_t version, replicas;
   uint64_t n_total, cells_total, list_total, prof_total, n_bricks, steps;
   uint32_t primed, hc_samples, hc_cap;
-  uint32_t sizeof_replica, sizeof_control, sizeof_settings, sizeof_brick, pad;
+  uint32_t sizeof_replica, sizeof_control, sizeof_settings, sizeof_brick, n_dense;
 };
 
 struct FileOut {
@@ -1629,6 +1651,7 @@ int vlj_save(vlj_sim* sim, const char* path) {
   h.sizeof_control = sizeof(Control);
   h.sizeof_settings = sizeof(vlj_settings);
   h.sizeof_brick = sizeof(Brick);
+  h.n_dense = sim->n_dense;
   w.put(&h, sizeof h);
   for (const HostReplica& x : sim->host) w.put(&x.settings, sizeof(vlj_settings));
   w.put(sim->rep.data(), R * sizeof(Replica));
@@ -1636,6 +1659,7 @@ int vlj_save(vlj_sim* sim, const char* path) {
     w.put(&x.prof_samples, sizeof(uint64_t));
     w.put(x.brick_x.data(), x.brick_x.size() * sizeof(uint32_t));
     w.put(x.brick_y.data(), x.brick_y.size() * sizeof(uint32_t));
+    w.put(x.sparse.data(), x.sparse.size());
   }
   w.put(sim->bricks.data(), nb * sizeof(Brick));
   int rc = save_array(sim, w, sim->B.ctl, R);
@@ -1749,10 +1773,14 @@ int vlj_load(const char* path, vlj_sim** out) {
   if (atoms != h.n_total || cells != h.cells_total) return bad("inconsistent sizes");
   std::vector<uint64_t> prof_samples(R);
   std::vector<std::vector<uint32_t>> bx(R), by(R);
+  std::vector<std::vector<uint8_t>> sp(R);
   for (size_t k = 0; k < R; k++) {
     r.get(&prof_samples[k], sizeof(uint64_t));
-    if (!read_vec(r, bx[k], rep[k].cells[2]) || !read_vec(r, by[k], rep[k].cells[2])) return bad("truncated");
+    if (!read_vec(r, bx[k], rep[k].cells[2]) || !read_vec(r, by[k], rep[k].cells[2]) ||
+        !read_vec(r, sp[k], rep[k].cells[2]))
+      return bad("truncated");
   }
+  if (h.n_dense > h.n_bricks) return bad("bad brick table");
   std::vector<Brick> bricks;
   std::vector<Control> ctl;
   StateArrays a;
@@ -1821,11 +1849,14 @@ int vlj_load(const char* path, vlj_sim** out) {
     B.n_bricks = (uint32_t)nb;
   }
   s->bricks = bricks;
+  s->n_dense = h.n_dense;
+  B.n_dense = h.n_dense;
   s->rep = rep;
   for (size_t k = 0; k < R; k++) {
     s->host[k].prof_samples = prof_samples[k];
     s->host[k].brick_x = bx[k];
     s->host[k].brick_y = by[k];
+    s->host[k].sparse = sp[k];
   }
   s->any_heat_exchange = false;
   for (const Replica& x : rep) s->any_heat_exchange = s->any_heat_exchange || x.hex_on;

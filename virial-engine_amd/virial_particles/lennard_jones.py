@@ -616,10 +616,10 @@ class LennardJones:
 
 
 def _debug_counters(h) -> dict:
-    out = np.zeros(5, dtype=np.uint64)
+    out = np.zeros(6, dtype=np.uint64)
     _check(L.lib().vlj_debug_counters(h, out.ctypes.data_as(C.POINTER(C.c_uint64))), h)
     return {"rebuilds": int(out[0]), "stage_overflow_builds": int(out[1]), "list_overflow": int(out[2]),
-            "bricks": int(out[3]), "global_cell_builds": int(out[4])}
+            "bricks": int(out[3]), "global_cell_builds": int(out[4]), "sparse_bricks": int(out[5])}
 
 
 @_render_methods
