@@ -192,17 +192,24 @@ __host__ __device__ __forceinline__ uint32_t dealt_index(uint32_t b) {
   return ((j / kChunk) * 8u + xcd) * kChunk + j % kChunk;
 }
 
-// A list entry is the neighbour's index in its brick's staging order (the
-// staged cells in (z, y, x) order, each cell's slots in cell order), 16 bits.
+// A list entry is 8 e, where e is the neighbour's index in its brick's
+// staging order (the staged cells in (z, y, x) order, each cell's slots in
+// cell order), 16 bits: the byte offset of its position in each of the
+// staged coordinate arrays, so the force loop addresses LDS with the entry
+// itself (kStageMax * 8 < 2^16).
 // Lists are stored in octets of 8 entries (16 bytes), transposed within
 // blocks of 64 consecutive atoms (one wave): octet o of the replica's atom li
 // is at (u16 units)
 //   list_off + (li / 64) * octets * 512 + o * 512 + (li % 64) * 8,
 // octets = ceil(capacity / 8). A wave reads octet o of its 64 atoms as 1 KB
 // of contiguous memory; all of a wave's lists lie in octets KB. The last
-// octet of a list is filled up with the layout's kListPad, the staged index of a sentinel
-// position far outside every box, so the force loop can take whole octets
-// without a per-entry bound.
+// octet of a list is filled up with the entry of the layout's kListPad, the staged index of a
+// sentinel position far outside every box, so the force loop can take whole
+// octets without a per-entry bound.
+constexpr uint32_t kEntryShift = 3;
+static_assert((kStageMax + 1u) << kEntryShift <= 0x10000u, "list entries are 16 bits");
+__host__ __device__ __forceinline__ uint32_t entry_of(uint32_t e) { return e << kEntryShift; }
+__host__ __device__ __forceinline__ uint32_t staged_of(uint32_t entry) { return entry >> kEntryShift; }
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr double kFar = 1.0e18;  // sentinel coordinate: r^2 ~ 3e36, beyond any cutoff
 

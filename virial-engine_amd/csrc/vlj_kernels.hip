@@ -594,15 +594,17 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Buffers B) {
 // start[c] .. start[c] + (off[c + 1] - off[c]).
 template <class L>
 struct Stage {
+  // Staged positions and, at L::kListPad, the sentinel. They open the
+  // structure and are a multiple of 64 doubles long, so that x[e] and y[e]
+  // are one ds_read2st64_b64 and z[e] one ds_read_b64, all three addressed
+  // by 8 e with no further offset.
+  double x[L::kStageMax + 64], y[L::kStageMax + 64], z[L::kStageMax + 64];
   uint32_t start[kStageCells];
   uint32_t off[kStageCells + 1];
   uint32_t hoff[kHomeCells + 1];  // home atoms numbered home cell by home cell
   uint8_t image[kStageCells];     // (ix + 1) + 3 (iy + 1) + 9 (iz + 1), i in {-1, 0, 1}
   uint32_t ns;                    // staged cells
   uint32_t total;                 // staged atoms
-  // Staged positions and, at L::kListPad, the sentinel. The arrays are a
-  // multiple of 64 doubles long so that x[e] and y[e] are one ds_read2st64_b64.
-  double x[L::kStageMax + 64], y[L::kStageMax + 64], z[L::kStageMax + 64];
 };
 
 // Staged index of the cell at offset (dz, yy, xx) from the brick's corner
@@ -1089,7 +1091,7 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
             if (mask != 0ull) {
               const uint32_t pos = f + lane_rank(mask);
               if (near && lane != sl && pos < cap)
-                *(uint16_t*)(lst + (size_t)(((pos & ~7u) << 7) | ((pos & 7u) << 1))) = (uint16_t)q[c];
+                *(uint16_t*)(lst + (size_t)(((pos & ~7u) << 7) | ((pos & 7u) << 1))) = (uint16_t)entry_of(q[c]);
               f += (uint32_t)__builtin_popcountll(mask);
             }
           }
@@ -1104,7 +1106,7 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
       const uint32_t li = lbase + k0 + lane;
       const uint32_t len = found < cap ? found : cap;
       uint16_t* lst = B.nbr + list_base(R, li);
-      for (uint32_t p = len; (p & 7u) != 0u; p++) lst[list_entry(p)] = (uint16_t)L::kListPad;
+      for (uint32_t p = len; (p & 7u) != 0u; p++) lst[list_entry(p)] = (uint16_t)entry_of(L::kListPad);
       B.nbr_count[R.atom_off + li] = len;
       if (found > cap) atomicMax(&B.ctl[g.rep].overflow, found);
     }
@@ -1201,7 +1203,20 @@ template <int PREC, bool SF, bool ENERGIES>
 __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, double dy, double dz,
                                                 double r2, PairAcc& a) {
   const bool hit = r2 < R.rc2;
-  if (PREC == 0) {
+  if (PREC == 0 && !SF && !ENERGIES) {
+    // The plain force takes the cutoff through r^2: a pair outside it gets
+    // 0x567 (2^360) as r^2's sign and exponent, so 1/r^2 is a normal number
+    // below 2^-360, s^6 rounds to exactly 0 (under 2^-1080), g is -0 and so is
+    // fscal, and the three fma() return their sums unchanged. One select of a
+    // 32-bit word instead of two after the arithmetic.
+    const int hi = hit ? __double2hiint(r2) : 0x56700000;
+    const double inv2 = rcp_rn(__hiloint2double(hi, __double2loint(r2)));
+    const double s6 = inv2 * inv2 * inv2;
+    const double fscal = s6 * __builtin_fma(s6, 2.0, -1.0) * inv2;
+    a.ax = __builtin_fma(dx, fscal, a.ax);
+    a.ay = __builtin_fma(dy, fscal, a.ay);
+    a.az = __builtin_fma(dz, fscal, a.az);
+  } else if (PREC == 0) {
     const double inv2 = rcp_rn(r2);
     const double s6 = inv2 * inv2 * inv2;
     if (!ENERGIES) {
@@ -1259,9 +1274,11 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
 #endif
 constexpr int kOctetBatch = VLJ_OCTET_BATCH;  // list octets (8 entries each) in flight per batch
 
-__device__ __forceinline__ u32x4 load_octet(const uint16_t* base, uint32_t o, uint32_t no) {
-  if (o < no) return __builtin_nontemporal_load((const u32x4*)(base + (uint64_t)o * (8u * kListStride)));
-  return u32x4{0u, 0u, 0u, 0u};
+// Octet o of a list, o clamped to the last octet of the list's capacity
+// (every atom's list owns all of them), so the load needs no branch; octets
+// past an atom's count are loaded but never summed.
+__device__ __forceinline__ u32x4 load_octet(const uint16_t* base, uint32_t o, uint32_t last) {
+  return __builtin_nontemporal_load((const u32x4*)(base + (uint64_t)min(o, last) * (8u * kListStride)));
 }
 
 // Atom i's list in order, kOctetBatch octets at a time, double-buffered:
@@ -1275,13 +1292,14 @@ __device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const R
   const uint32_t cnt = B.nbr_count[i];
   const uint32_t no = (cnt + 7u) / 8u;
   const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
+  const uint32_t last = list_octets(R.capacity) - 1u;
   PairAcc a = {0.0, 0.0, 0.0, 0.0, 0.0, 0u, 0u};
   u32x4 cur[kOctetBatch], nxt[kOctetBatch];
 #pragma unroll
-  for (int b = 0; b < kOctetBatch; b++) cur[b] = load_octet(lst, (uint32_t)b, no);
+  for (int b = 0; b < kOctetBatch; b++) cur[b] = load_octet(lst, (uint32_t)b, last);
   for (uint32_t o0 = 0; o0 < no; o0 += kOctetBatch) {
 #pragma unroll
-    for (int b = 0; b < kOctetBatch; b++) nxt[b] = load_octet(lst, o0 + kOctetBatch + (uint32_t)b, no);
+    for (int b = 0; b < kOctetBatch; b++) nxt[b] = load_octet(lst, o0 + kOctetBatch + (uint32_t)b, last);
 #pragma unroll
     for (int b = 0; b < kOctetBatch; b++) {
       if (o0 + (uint32_t)b < no) {
@@ -1289,10 +1307,10 @@ __device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const R
         double dx[8], dy[8], dz[8];
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-          const uint32_t e = (w[c >> 1] >> ((c & 1) * 16)) & 0xffffu;
-          dx[c] = S.x[e] - xi;
-          dy[c] = S.y[e] - yi;
-          dz[c] = S.z[e] - zi;
+          const uint32_t off = (w[c >> 1] >> ((c & 1) * 16)) & 0xffffu;  // 8 e
+          dx[c] = *(const double*)((const char*)S.x + off) - xi;
+          dy[c] = *(const double*)((const char*)S.y + off) - yi;
+          dz[c] = *(const double*)((const char*)S.z + off) - zi;
         }
 #pragma unroll
         for (int c = 0; c < 8; c++)
@@ -1315,7 +1333,7 @@ __device__ __forceinline__ PairAcc interactions_global(const Buffers& B, const R
   PairAcc a = {0.0, 0.0, 0.0, 0.0, 0.0, 0u, 0u};
   for (uint32_t k = 0; k < cnt; k++) {
     double xj, yj, zj;
-    staged_pos(R, S, false, lst[list_entry(k)], B.x, B.y, B.z, xj, yj, zj);
+    staged_pos(R, S, false, staged_of(lst[list_entry(k)]), B.x, B.y, B.z, xj, yj, zj);
     const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
     pair_accumulate<PREC, SF, ENERGIES>(R, dx, dy, dz, dist2(dx, dy, dz), a);
   }
@@ -1516,7 +1534,7 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
     const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
     for (uint32_t k = 0; k < cnt; k++) {
       double xj, yj, zj;
-      staged_pos(R, S, staged, lst[list_entry(k)], B.x, B.y, B.z, xj, yj, zj);
+      staged_pos(R, S, staged, staged_of(lst[list_entry(k)]), B.x, B.y, B.z, xj, yj, zj);
       const double d[3] = {xj - xi, yj - yi, zj - zi};
       const double r2 = dist2(d[0], d[1], d[2]);
       if (!(r2 < R.rc2)) continue;
@@ -1636,7 +1654,7 @@ __device__ __forceinline__ void heat_part(const Buffers& B, const Replica& R, co
     const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
     for (uint32_t k = 0; k < cnt; k++) {
       double xj, yj, zj;
-      staged_pos(R, S, staged, lst[list_entry(k)], B.x, B.y, B.z, xj, yj, zj);
+      staged_pos(R, S, staged, staged_of(lst[list_entry(k)]), B.x, B.y, B.z, xj, yj, zj);
       const double d[3] = {xj - xi, yj - yi, zj - zi};
       const double r2 = dist2(d[0], d[1], d[2]);
       if (!(r2 < R.rc2)) continue;
@@ -1712,7 +1730,7 @@ __device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, 
     const uint32_t cnt = B.nbr_count[i];
     const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
     for (uint32_t k = 0; k < cnt; k++) {
-      const uint32_t e = lst[list_entry(k)];
+      const uint32_t e = staged_of(lst[list_entry(k)]);
       double xj, yj, zj;
       staged_pos(R, S, staged, e, B.x, B.y, B.z, xj, yj, zj);
       const double r2 = dist2(xj - xi, yj - yi, zj - zi);

@@ -1167,7 +1167,7 @@ int vlj_debug_lists(vlj_sim* sim, uint32_t* counts, uint32_t* entries) {
   if (rc != VLJ_OK) return rc;
   if (counts) std::memcpy(counts, cnt.data(), cnt.size() * 4);
   if (entries) {
-    // Entries are staged indices of the atom's brick; rebuild each brick's
+    // Entries are (8 x) staged indices of the atom's brick; rebuild each brick's
     // staging order from the cell tables to turn them into slots.
     const size_t padded = (size_t)((R.count + kListStride - 1) / kListStride) * kListStride;
     std::vector<uint16_t> t((size_t)list_octets(R.capacity) * 8u * padded);
@@ -1201,7 +1201,7 @@ int vlj_debug_lists(vlj_sim* sim, uint32_t* counts, uint32_t* entries) {
       for (uint32_t k = 0; k < R.capacity; k++) {
         uint32_t slot = 0;
         if (k < cnt[i]) {
-          const uint32_t e = t[base + list_entry(k)];
+          const uint32_t e = staged_of(t[base + list_entry(k)]);
           size_t sc = std::upper_bound(off.begin(), off.end(), e) - off.begin() - 1;
           while (sc + 1 < off.size() && off[sc + 1] <= e) sc++;
           slot = sorted[first[sc] + (e - off[sc])] - R.atom_off;
