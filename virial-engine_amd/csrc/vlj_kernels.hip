@@ -1070,30 +1070,31 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
         }
       }
       const uint32_t nc = (T - c0 + 63u) / 64u;  // chunks of this group that hold candidates
+      uint32_t qe[kGroup];
+#pragma unroll
+      for (int c = 0; c < kGroup; c++) qe[c] = entry_of(q[c]);
       for (uint32_t k = k0; k < k1; k++) {
         const uint32_t e = ebase + k;
         double xi, yi, zi;
         staged_pos(R, S, staged, e, X, Y, Z, xi, yi, zi);
         uint32_t f = readlane_u32(found, k - k0);
         char* lst = (char*)(B.nbr + list_base(R, lbase + k));
-        // the atom itself is candidate number e - rdelta[4] (the home cell is
-        // the middle of run 4, dz = dy = 1): its lane in chunk c, if any
-        const uint32_t self = e - rdelta[4] - c0;
 #pragma unroll
         for (int c = 0; c < kGroup; c++) {
           if ((uint32_t)c < nc) {
-            // a ballot marks the candidates within rc + skin (not the atom
-            // itself); each hit goes at the list length plus its rank
-            const bool near = dist2(cx[c] - xi, cy[c] - yi, cz[c] - zi) < R.list_r2;
-            const uint32_t sl = self - 64u * (uint32_t)c;  // the atom's own lane, if below 64
+            // a ballot marks the candidates within rc + skin other than the
+            // atom itself (the one candidate whose staged index is e); each
+            // hit goes at the list length plus its rank
+            const double d2 = dist2(cx[c] - xi, cy[c] - yi, cz[c] - zi);
+            const bool near = d2 < R.list_r2 && q[c] != e;
+            // the same predicate as a lane mask, straight from the compares
+            // (ordered less-than, unsigned not-equal)
             const unsigned long long mask =
-                __builtin_amdgcn_ballot_w64(near) & ~(sl < 64u ? 1ull << sl : 0ull);
-            if (mask != 0ull) {
-              const uint32_t pos = f + lane_rank(mask);
-              if (near && lane != sl && pos < cap)
-                *(uint16_t*)(lst + (size_t)(((pos & ~7u) << 7) | ((pos & 7u) << 1))) = (uint16_t)entry_of(q[c]);
-              f += (uint32_t)__builtin_popcountll(mask);
-            }
+                __builtin_amdgcn_fcmp(d2, R.list_r2, 4) & __builtin_amdgcn_uicmp(q[c], e, 33);
+            const uint32_t pos = f + lane_rank(mask);
+            if (near && pos < cap)
+              *(uint16_t*)(lst + (size_t)(((pos & ~7u) << 7) | ((pos & 7u) << 1))) = (uint16_t)qe[c];
+            f += (uint32_t)__builtin_popcountll(mask);
           }
         }
         found = lane == k - k0 ? f : found;
