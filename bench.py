@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # Algorithmic bytes of one force-kernel launch per atom: read x_i and v_i,
 # write a_i and v_i, 24 bytes each in FP64.
 FORCE_BYTES_PER_ATOM = 96.0
-FORCE_KERNEL = "k_force<0, false, false>"
+FORCE_KERNEL = "k_force<Layout<512, 3200, false>, 0, false, false>"
 COPY_KERNEL = "k_copy"
 COPY_BYTES = 1 << 30
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_fcc1m_traffic.txt")
@@ -475,7 +475,7 @@ def main():
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic_bytes,
-        "kernel": "k_force<0, false, false>: FP64 pair forces from brick-staged lists + second half kick",
+        "kernel": FORCE_KERNEL + ": FP64 pair forces from brick-staged lists + second half kick",
         "algorithmic_bytes_per_launch": FORCE_BYTES_PER_ATOM * n,
         "avg_launch_us": fp64["force_kernel_us"],
         "measured_copy_GBs": copy_gbs,

@@ -1275,9 +1275,10 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
 #endif
 constexpr int kOctetBatch = VLJ_OCTET_BATCH;  // list octets (8 entries each) in flight per batch
 
-// Octet o of a list, o clamped to the last octet of the list's capacity
-// (every atom's list owns all of them), so the load needs no branch; octets
-// past an atom's count are loaded but never summed.
+// Octet o of a list, o clamped to `last`, the list's own last octet (its
+// first when it is empty), so the load needs no branch and a clamped load
+// hits the cache line just read; octets past an atom's count are loaded but
+// never summed.
 __device__ __forceinline__ u32x4 load_octet(const uint16_t* base, uint32_t o, uint32_t last) {
   return __builtin_nontemporal_load((const u32x4*)(base + (uint64_t)min(o, last) * (8u * kListStride)));
 }
@@ -1293,7 +1294,7 @@ __device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const R
   const uint32_t cnt = B.nbr_count[i];
   const uint32_t no = (cnt + 7u) / 8u;
   const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
-  const uint32_t last = list_octets(R.capacity) - 1u;
+  const uint32_t last = no > 0u ? no - 1u : 0u;
   PairAcc a = {0.0, 0.0, 0.0, 0.0, 0.0, 0u, 0u};
   u32x4 cur[kOctetBatch], nxt[kOctetBatch];
 #pragma unroll
