@@ -1055,11 +1055,12 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
       for (int c = 0; c < kGroup; c++) {
         const uint32_t a = c0 + 64u * (uint32_t)c, s = a + lane;
         q[c] = L::kListPad;
+        cx[c] = cy[c] = cz[c] = kFar;
+        if (a >= T) continue;  // chunks past the candidates stay unread (a dilute item fills one)
 #pragma unroll
         for (int r = 8; r >= 0; r--)  // the first run that ends past s
           if (rend[r] > a && (r == 0 || rend[r - 1] < a + 64u) && s < rend[r]) q[c] = s + rdelta[r];
         const bool valid = s < T;
-        cx[c] = cy[c] = cz[c] = kFar;
         if (staged) {
           const uint32_t qs = valid ? q[c] : L::kListPad;
           cx[c] = S.x[qs];
