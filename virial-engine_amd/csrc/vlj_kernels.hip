@@ -91,11 +91,18 @@ __global__ void __launch_bounds__(kBlock) k_decide(Buffers B, int advance) {
   const uint32_t r = blockIdx.x;
   const Replica R = B.rep[r];
   unsigned long long m = 0ull;
-  if (advance)
-    for (uint32_t b = R.atom_block_off + threadIdx.x; b < R.atom_block_off + R.atom_blocks; b += kBlock) {
-      const unsigned long long v = B.bmax[b];
-      m = v > m ? v : m;
+  if (advance) {
+    // 16 loads in flight per thread (4096 workgroup maxima of a 1M-atom
+    // replica in one round trip)
+    const uint32_t end = R.atom_block_off + R.atom_blocks;
+    for (uint32_t b0 = R.atom_block_off + threadIdx.x; b0 < end; b0 += 16u * kBlock) {
+      unsigned long long v[16];
+#pragma unroll
+      for (uint32_t j = 0; j < 16u; j++) v[j] = b0 + j * kBlock < end ? B.bmax[b0 + j * kBlock] : 0ull;
+#pragma unroll
+      for (uint32_t j = 0; j < 16u; j++) m = v[j] > m ? v[j] : m;
     }
+  }
   __shared__ unsigned long long wmax[kBlock / 64];
   m = wave_max_u64(m);
   if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
