@@ -142,3 +142,22 @@ def test_fp32_forces_against_fp64_oracle(oracle, lj):
     assert rel_rms < 1e-6
     assert abs((m.potential - r.potential) / r.potential) < 4e-7
     assert abs((m.virial - r.virial) / r.virial) < 4e-7
+
+
+@pytest.mark.parametrize("rho", [0.01, 0.05, 0.2])
+def test_dilute_lists_match_the_oracle(oracle, lj, rho):
+    # dilute systems (sparse bricks, cells of one or a few atoms, items of
+    # one atom): every list holds the oracle's entries in the oracle's
+    # order, before and after rebuilds
+    n = 3000
+    side = float(np.float32((n / rho) ** (1.0 / 3.0)))
+    parts = lj.random_insertion_array(n, [side] * 3, 0.9, 1.5, 29)
+    pos, vel = np.ascontiguousarray(parts["pos"]), np.ascontiguousarray(parts["vel"])
+    sim, ref = make_pair(oracle, lj, pos, vel, side)
+    assert list(sim.measure_now().raw) == list(ref.measure().raw)  # first lists built
+    assert_same_grid(sim, ref)
+    sim.run(200)
+    ref.run(200)
+    assert sim.debug_counters()["rebuilds"] > 1
+    assert_same_grid(sim, ref)
+    assert list(sim.measure_now().raw) == list(ref.measure().raw)
