@@ -304,6 +304,12 @@ int vlj_debug_rcp(const double *x, double *out, size_t n);
  * events. Measures the bandwidth a streaming kernel reaches and calibrates
  * rocprof byte counters (no counterpart in the reference). */
 int vlj_copy_bandwidth(size_t bytes, uint32_t repeats, double *ms);
+/* FP64 vector rate the device reaches: chains of dependent fused
+ * multiply-adds, eight independent chains a thread over a grid that fills
+ * every CU, `repeats` launches timed with HIP events; *tflops = 2 flops per
+ * FMA / time. The measured peak beside the published one for the force
+ * kernel's VALU roofline (no counterpart in the reference). */
+int vlj_fp64_fma_rate(uint32_t repeats, double *tflops);
 
 /* Kernel timing with HIP events on the handle's stream: runs `steps` steps
  * launching the kernels directly (not from the captured graph) with an event

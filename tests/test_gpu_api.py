@@ -246,3 +246,11 @@ def test_empty_and_tiny_systems(lj):
     assert m.potential == 0.0
     p = one.read_particles()[0]
     assert p.pos[0] == pytest.approx(1.0 + 100 * 0.005 * 0.1)
+
+
+def test_fp64_fma_rate_is_plausible(lj):
+    # the FMA-chain measurement that the force kernel's VALU roofline sits
+    # beside: positive and not above MI355X's published 78.6 TFLOP/s by
+    # more than timing noise
+    tf = lj.fp64_fma_rate(3)
+    assert 20.0 < tf < 1.2 * 78.6

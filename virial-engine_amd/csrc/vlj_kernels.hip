@@ -1820,6 +1820,24 @@ __global__ void __launch_bounds__(kBlock) k_copy(const u32x4* __restrict__ src, 
     __builtin_nontemporal_store(__builtin_nontemporal_load(src + t), dst + t);
 }
 
+// FP64 FMA chains: eight independent chains per thread, `iters` dependent
+// fused multiply-adds each (2 flops), the eight results summed and stored
+// so that none is dropped. Measures the FP64 vector rate the device reaches.
+__global__ void __launch_bounds__(kBlock) k_fma(double* out, uint32_t iters, double m, double c) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  double a[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) a[j] = 1.0 + 1e-9 * (double)t + 0.125 * (double)j;
+  for (uint32_t k = 0; k < iters; k++) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[j] = __builtin_fma(a[j], m, c);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) s += a[j];
+  out[t] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -2021,6 +2039,11 @@ hipError_t launch_fixed_sum(const double* values, size_t n, double scale, long l
     const uint32_t blocks = (uint32_t)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_fixed_sum, grid1(blocks), dim3(kBlock), 0, s, values, n, scale, out);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_fma(double* out, uint32_t blocks, uint32_t iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_fma, grid1(blocks), dim3(kBlock), 0, s, out, iters, 0.999999999, 1e-12);
   return hipGetLastError();
 }
 

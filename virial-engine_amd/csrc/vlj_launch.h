@@ -27,5 +27,6 @@ hipError_t launch_rcp(const double* x, double* out, size_t n, hipStream_t s);
 hipError_t launch_fixed_sum(const double* values, size_t n, double scale, long long* out,
                             hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t words, hipStream_t s);
+hipError_t launch_fma(double* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 
 }  // namespace vlj

@@ -1291,6 +1291,36 @@ int vlj_copy_bandwidth(size_t bytes, uint32_t repeats, double* ms) {
   return VLJ_OK;
 }
 
+int vlj_fp64_fma_rate(uint32_t repeats, double* tflops) {
+  if (!tflops || repeats == 0) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_fp64_fma_rate");
+  vlj_sim tmp;
+  int rc = check_device(&tmp);
+  if (rc != VLJ_OK) return rc;
+  const uint32_t blocks = 256u * 16u, iters = 256u;  // 1M threads x 8 chains x 256 FMAs
+  double* out = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&out, (size_t)blocks * kBlock * sizeof(double));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_fma(out, blocks, iters, st);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (uint32_t k = 0; k < repeats && e == hipSuccess; k++) e = launch_fma(out, blocks, iters, st);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (out) (void)hipFree(out);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (e != hipSuccess) return fail(nullptr, VLJ_E_HIP, std::string("vlj_fp64_fma_rate: ") + hipGetErrorString(e));
+  const double flops = 2.0 * 8.0 * iters * (double)blocks * kBlock * repeats;
+  *tflops = t > 0.0f ? flops / ((double)t * 1e-3) / 1e12 : 0.0;
+  return VLJ_OK;
+}
+
 int vlj_debug_rcp(const double* x, double* out, size_t n) {
   if (n && (!x || !out)) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_debug_rcp");
   vlj_sim tmp;

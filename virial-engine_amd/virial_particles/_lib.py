@@ -135,6 +135,7 @@ SIGNATURES = {
     "vlj_debug_counters": (C.c_int, [_P, _u64p]),
     "vlj_debug_swizzle": (_u32, [_u32, _u32]),
     "vlj_copy_bandwidth": (C.c_int, [_sz, _u32, _dp]),
+    "vlj_fp64_fma_rate": (C.c_int, [_u32, _dp]),
     "vlj_debug_rcp": (C.c_int, [_dp, _dp, _sz]),
     "vlj_timed_run": (C.c_int, [_P, _u64, _dp, _dp]),
 }

@@ -758,6 +758,13 @@ def copy_bandwidth(nbytes: int, repeats: int = 10) -> float:
     return ms.value
 
 
+def fp64_fma_rate(repeats: int = 10) -> float:
+    """TFLOP/s of FP64 fused multiply-add chains over the whole device."""
+    tf = C.c_double()
+    _check(L.lib().vlj_fp64_fma_rate(repeats, C.byref(tf)))
+    return tf.value
+
+
 def fixed_sum(values, fixed_scale: float) -> int:
     """Device 64-bit fixed-point sum of round-ties-even(values * fixed_scale)."""
     v = np.ascontiguousarray(values, dtype=np.float64)
