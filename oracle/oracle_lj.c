@@ -550,6 +550,13 @@ int vo_derive_params(const double *pos, size_t n, const double box_in[3], double
     double local = (double)occ_max / cell_volume;
     double use = local > density ? local : density;
     p->capacity = (uint32_t)ceil(2.0 * sphere * use + 32.0);
+    /* VLJ_LIST_CAPACITY, when set, replaces the rule (at least 8): the tests'
+     * way to reach list overflow, which the rule avoids for the initial state */
+    const char *cap_env = getenv("VLJ_LIST_CAPACITY");
+    if (cap_env) {
+        int c = atoi(cap_env);
+        p->capacity = (uint32_t)(c > 8 ? c : 8);
+    }
 
     double kinetic = 0.0, speed = 0.0;
     for (size_t i = 0; i < n; i++) {

@@ -161,3 +161,20 @@ def test_dilute_lists_match_the_oracle(oracle, lj, rho):
     assert sim.debug_counters()["rebuilds"] > 1
     assert_same_grid(sim, ref)
     assert list(sim.measure_now().raw) == list(ref.measure().raw)
+
+
+def test_list_overflow_matches_the_oracle(oracle, lj, monkeypatch):
+    # lennard_jones.wgsl:494-497: past the capacity a list keeps its first
+    # `capacity` entries and the measurement reports the longest list found;
+    # a capacity of 40 (the rule gives ~190 here) overflows every list
+    monkeypatch.setenv("VLJ_LIST_CAPACITY", "40")
+    pos, vel = jittered_lattice(oracle)
+    sim, ref = make_pair(oracle, lj, pos, vel, 11.0, dt=0.002)
+    for steps in (0, 50):
+        sim.run(steps)
+        ref.run(steps)
+        m, r = sim.measure_now(), ref.measure()
+        assert r.list_overflow > 40
+        assert m.list_overflow == r.list_overflow
+        assert list(m.raw) == list(r.raw)
+        assert_same_grid(sim, ref)

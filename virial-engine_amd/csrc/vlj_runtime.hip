@@ -183,6 +183,10 @@ int derive(const vlj_particle* p, size_t n, const vlj_settings& st, Replica& R, 
   const double cell_volume = cell_size[0] * cell_size[1] * cell_size[2];
   const double use = std::max((double)occ_max / cell_volume, density);
   R.capacity = (uint32_t)std::ceil(2.0 * sphere * use + 32.0);
+  // VLJ_LIST_CAPACITY, when set, replaces the rule (at least 8): tests use a
+  // small capacity to reach list overflow, which the rule itself avoids for
+  // the initial state
+  if (const char* c = std::getenv("VLJ_LIST_CAPACITY")) R.capacity = (uint32_t)std::max(8, std::atoi(c));
 
   double kinetic = 0.0, speed = 0.0;
   for (size_t i = 0; i < n; i++) {
