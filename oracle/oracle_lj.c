@@ -33,8 +33,10 @@
  *     over the list, and a = -24 G at the end (lennard_jones.wgsl:501-545,
  *     with the factor 24 of r.F = 24 s6 (2 s6 - 1) taken out of the sum).
  *  R5f pair, FP32 forces: cutoff test and d as R5 in FP64, then in f32:
- *     q = (float)r2, inv2 = 1/q, s6, g, fscal as R5; (float)d * fscal
- *     accumulated in FP64, a = -24 G.
+ *     q = (float)r2, inv2 = 1/q, s6, g, fscal as R5; f = fmaf((float)d,
+ *     fscal, f) in f32 over each octet of the list (entries 8m .. 8m + 7,
+ *     pairs past the cutoff adding nothing), each octet's f added to G in
+ *     FP64, a = -24 G.
  *  R9 energies: rf = s6 fma(48, s6, -24); u = fma(4 s6, s6, -4 s6) - shift;
  *     shifted force: r = sqrt(r2), rf = fma(-r, fs, rf), u = fma(r - rc, fs, u).
  *  R6 drift (velocity Verlet): v = fma(a, dt/2, v); x = fma(v, dt, x);
@@ -830,6 +832,7 @@ static void interactions(vo_sim *s, int32_t i, double acc[3], int energies)
     const double fs24 = p->force_shift / 24.0;
     double xi = s->x[0][i], yi = s->x[1][i], zi = s->x[2][i];
     double gx = 0.0, gy = 0.0, gz = 0.0, pe = 0.0, w = 0.0;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f; /* R5f: the current octet's sum */
     uint32_t near = 0, within = 0;
     uint32_t cnt = s->nbr_count[i];
     const uint32_t *row = s->nbr + (size_t)i * p->capacity;
@@ -845,7 +848,7 @@ static void interactions(vo_sim *s, int32_t i, double acc[3], int energies)
         double dz = (s->x[2][j] + sz) - zi;
         double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (!(r2 < p->rc2))
-            continue;
+            goto next; /* past the cutoff: no term (an R5f octet still ends) */
         double rf = 0.0, u = 0.0;
         if (p->precision == 0) {
             double inv2 = 1.0 / r2;
@@ -875,9 +878,9 @@ static void interactions(vo_sim *s, int32_t i, double acc[3], int energies)
             if (p->shifted_force)
                 g = fmaf(-sqrtf(q), (float)fs24, g);
             float fscal = g * inv2;
-            gx += (double)((float)dx * fscal);
-            gy += (double)((float)dy * fscal);
-            gz += (double)((float)dz * fscal);
+            fx = fmaf((float)dx, fscal, fx);
+            fy = fmaf((float)dy, fscal, fy);
+            fz = fmaf((float)dz, fscal, fz);
             if (energies) {
                 float rff = s6 * fmaf(48.0f, s6, -24.0f);
                 float t = 4.0f * s6;
@@ -897,6 +900,13 @@ static void interactions(vo_sim *s, int32_t i, double acc[3], int energies)
             w += 0.5 * rf;
             near += r2 < 2.25 ? 1u : 0u;
             within += 1u;
+        }
+    next:
+        if (p->precision != 0 && ((k & 7u) == 7u || k + 1u == cnt)) {
+            gx += (double)fx;
+            gy += (double)fy;
+            gz += (double)fz;
+            fx = fy = fz = 0.0f;
         }
     }
     acc[0] = -24.0 * gx;

@@ -1248,17 +1248,11 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
       a.w += hit ? 0.5 * rf : 0.0;
     }
   } else {
+    static_assert(PREC == 0 || ENERGIES, "FP32 forces sum through pair_force_f32");
     const float q = (float)r2;
     const float inv2 = __builtin_amdgcn_rcpf(q);
     const float s6 = inv2 * inv2 * inv2;
-    if (!ENERGIES) {
-      float g = s6 * __builtin_fmaf(s6, 2.0f, -1.0f);
-      if (SF) g = __builtin_fmaf(-__builtin_sqrtf(q), (float)R.force_shift24, g);
-      const float fscal = hit ? g * inv2 : 0.0f;
-      a.ax += (double)((float)dx * fscal);
-      a.ay += (double)((float)dy * fscal);
-      a.az += (double)((float)dz * fscal);
-    } else {
+    {
       float rf = s6 * __builtin_fmaf(48.0f, s6, -24.0f);
       const float t = 4.0f * s6;
       float u = __builtin_fmaf(t, s6, -t) - (float)R.shift;
@@ -1276,6 +1270,24 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
     a.near += hit && r2 < kNearR2 ? 1u : 0u;
     a.within += hit ? 1u : 0u;
   }
+}
+
+// R5f's force: the pair factor in FP32 (on the FP64 cutoff test) and
+// f += (float)d fscal in FP32 over the eight entries of one list octet; the
+// caller adds each octet's f to the FP64 sum (pads add exactly 0).
+template <bool SF>
+__device__ __forceinline__ void pair_force_f32(const Replica& R, double dx, double dy, double dz, double r2,
+                                               float& fx, float& fy, float& fz) {
+  const bool hit = r2 < R.rc2;
+  const float q = (float)r2;
+  const float inv2 = __builtin_amdgcn_rcpf(q);
+  const float s6 = inv2 * inv2 * inv2;
+  float g = s6 * __builtin_fmaf(s6, 2.0f, -1.0f);
+  if (SF) g = __builtin_fmaf(-__builtin_sqrtf(q), (float)R.force_shift24, g);
+  const float fscal = hit ? g * inv2 : 0.0f;
+  fx = __builtin_fmaf((float)dx, fscal, fx);
+  fy = __builtin_fmaf((float)dy, fscal, fy);
+  fz = __builtin_fmaf((float)dz, fscal, fz);
 }
 
 #ifndef VLJ_OCTET_BATCH
@@ -1322,9 +1334,19 @@ __device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const R
           dy[c] = *(const double*)((const char*)S.y + off) - yi;
           dz[c] = *(const double*)((const char*)S.z + off) - zi;
         }
+        if constexpr (PREC == 1 && !ENERGIES) {
+          float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
-        for (int c = 0; c < 8; c++)
-          pair_accumulate<PREC, SF, ENERGIES>(R, dx[c], dy[c], dz[c], dist2(dx[c], dy[c], dz[c]), a);
+          for (int c = 0; c < 8; c++)
+            pair_force_f32<SF>(R, dx[c], dy[c], dz[c], dist2(dx[c], dy[c], dz[c]), fx, fy, fz);
+          a.ax += (double)fx;
+          a.ay += (double)fy;
+          a.az += (double)fz;
+        } else {
+#pragma unroll
+          for (int c = 0; c < 8; c++)
+            pair_accumulate<PREC, SF, ENERGIES>(R, dx[c], dy[c], dz[c], dist2(dx[c], dy[c], dz[c]), a);
+        }
       }
     }
 #pragma unroll
@@ -1341,11 +1363,22 @@ __device__ __forceinline__ PairAcc interactions_global(const Buffers& B, const R
   const uint32_t cnt = B.nbr_count[i];
   const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
   PairAcc a = {0.0, 0.0, 0.0, 0.0, 0.0, 0u, 0u};
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;  // FP32 forces: the current octet's sum
   for (uint32_t k = 0; k < cnt; k++) {
     double xj, yj, zj;
     staged_pos(R, S, false, staged_of(lst[list_entry(k)]), B.x, B.y, B.z, xj, yj, zj);
     const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
-    pair_accumulate<PREC, SF, ENERGIES>(R, dx, dy, dz, dist2(dx, dy, dz), a);
+    if constexpr (PREC == 1 && !ENERGIES) {
+      pair_force_f32<SF>(R, dx, dy, dz, dist2(dx, dy, dz), fx, fy, fz);
+      if ((k & 7u) == 7u || k + 1u == cnt) {
+        a.ax += (double)fx;
+        a.ay += (double)fy;
+        a.az += (double)fz;
+        fx = fy = fz = 0.0f;
+      }
+    } else {
+      pair_accumulate<PREC, SF, ENERGIES>(R, dx, dy, dz, dist2(dx, dy, dz), a);
+    }
   }
   return a;
 }
