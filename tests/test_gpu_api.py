@@ -254,3 +254,23 @@ def test_fp64_fma_rate_is_plausible(lj):
     # more than timing noise
     tf = lj.fp64_fma_rate(3)
     assert 20.0 < tf < 1.2 * 78.6
+
+
+def test_fp32_form_is_the_same_on_cut_bricks(oracle, lj, monkeypatch):
+    # the FP32 form sums each list octet in FP32 by list position, so bricks
+    # cut into parts (other staging, same list order) give the same bits
+    size = [24.0, 24.0, 12.0]
+    count = int(round(0.8442 * np.prod(size)))
+    pos, vel = oracle.lattice_box(count, size, 1.0, 5)
+    runs = []
+    for fill in (None, "100"):
+        if fill is None:
+            monkeypatch.delenv("VLJ_STAGE_FILL", raising=False)
+        else:
+            monkeypatch.setenv("VLJ_STAGE_FILL", fill)
+        sim = lj.LennardJones((pos, vel), lj.Settings(size, 2.5, 0.002, precision=1))
+        sim.run(40)
+        runs.append((list(sim.measure_now().raw), sim.debug_state()[0], sim.debug_counters()))
+    assert runs[0][2]["stage_overflow_builds"] == 0 and runs[1][2]["stage_overflow_builds"] > 0
+    assert runs[0][0] == runs[1][0]
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
