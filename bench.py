@@ -435,8 +435,8 @@ def main():
         traffic(args.traffic[0], args.traffic[1], args.out, args.command)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this path has no multi-GPU form "
-              "(DESIGN.md section 7 says how it would shard)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this path has no multi-GPU form; "
+              "replicas would be split across GPUs by cost, a single system into z slabs", file=sys.stderr)
         sys.exit(2)
 
     from virial_particles import lennard_jones as lj
@@ -487,8 +487,8 @@ def main():
         "measured_copy_GBs": copy_gbs,
     }
     # The same kernel against the FP64 vector peak: 22 flops per unique pair
-    # within rc (P_u from the pair counter), the per-atom-step work of the
-    # survey's convention; a full list evaluates each pair twice.
+    # within rc (P_u from the pair counter) per atom-step; a full list
+    # evaluates each pair twice, which this count does not credit.
     p_u = fp64["pairs_within_cutoff"] / n
     valu_achieved = PAIR_FLOPS * p_u * n / (fp64["force_kernel_us"] * 1e-6) / 1e12
     valu = {

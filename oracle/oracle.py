@@ -1,8 +1,8 @@
 """ctypes binding of the CPU oracle (liboracle_lj.so).
 
 TEST INFRASTRUCTURE ONLY: imported by the test suite, the smoke check and the
-CPU-baseline leg of the benchmark, as the checker. The product package never
-imports this module.
+CPU-baseline leg of the benchmark, to check the product's results. The
+product package never imports this module.
 """
 from __future__ import annotations
 

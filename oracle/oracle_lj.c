@@ -1,6 +1,7 @@
 /*
  * CPU f64 oracle for the Lennard-Jones hot path. TEST INFRASTRUCTURE ONLY:
- * the checker, never the thing measured or shipped. See oracle_lj.h.
+ * it produces expected answers and is never the thing measured or shipped.
+ * See oracle_lj.h.
  *
  * Built with -ffp-contract=off: every fused multiply-add below is written as
  * fma() on purpose, and nothing else is fused, so the HIP kernels (built the

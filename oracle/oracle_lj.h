@@ -3,9 +3,9 @@
  * (crates/virial-particles/src/lennard_jones.{rs,wgsl}, grid.wgsl,
  * crates/virial-precision/src/fixed.{rs,wgsl}).
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker: the test suite, the
- * smoke check and the CPU-baseline leg of the benchmark call it to produce the
- * expected answers. The product library never links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY. The test suite, the smoke check and the
+ * CPU-baseline leg of the benchmark call this library to produce the expected
+ * answers. The product library never links, loads or calls it.
  *
  * The reference is Rust + WGSL and has no CPU Lennard-Jones path of its own
  * (crates/virial-reference/src/lib.rs holds gravity only), so this file is a
