@@ -149,7 +149,7 @@ int64_t vo_fixed_to_raw(float x)
 /* fixed.rs:55-57 */
 double vo_fixed_to_f64(int64_t raw) { return (double)raw / TWO32; }
 
-/* The FP64 counterpart of fixed_split used by this build: round-ties-even
+/* The FP64 counterpart of fixed_split that the HIP library uses: round-ties-even
  * of x * fixed_scale, where fixed_scale = scale * 2^32 is a power of two, so
  * the product is exact and the only rounding is llrint's. */
 int64_t vo_fixed_f64(double x, double fixed_scale) { return llrint(x * fixed_scale); }
