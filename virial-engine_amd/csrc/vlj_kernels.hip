@@ -637,13 +637,17 @@ __device__ __forceinline__ void image_shift(const Replica& R, uint32_t code, dou
   sz = (double)((int)(code / 9u) - 1) * R.box[2];
 }
 
-// Wave priority while a brick is staged: with two workgroups on a CU, the
-// one staging its next brick issues its loads and LDS stores ahead of the
-// one summing pairs, so the CU spends less time with both waiting on
-// memory (FP64 force kernel at N = 1M: 165 -> 153 us; priorities 1 and 3
-// alike).
+// Wave priorities (s_setprio). While a brick is staged: with two workgroups
+// on a CU, the one staging its next brick issues its loads and LDS stores
+// ahead of the one summing pairs, so the CU spends less time with both
+// waiting on memory (FP64 force kernel at N = 1M: 165 -> 153 us; 1 and 3
+// alike). While the FP64 pair loop issues an octet's gathers: 1, below
+// staging.
 #ifndef VLJ_STAGE_PRIO
-#define VLJ_STAGE_PRIO 1
+#define VLJ_STAGE_PRIO 2
+#endif
+#ifndef VLJ_READ_PRIO
+#define VLJ_READ_PRIO 1
 #endif
 
 // Fills S for brick g from the positions X, Y, Z. Every thread of the
@@ -1342,6 +1346,10 @@ __device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const R
       if (o0 + (uint32_t)b < no) {
         const uint32_t w[4] = {cur[b].x, cur[b].y, cur[b].z, cur[b].w};
         double dx[8], dy[8], dz[8];
+        // the FP64 form issues an octet's LDS gathers at raised priority
+        // (150 against 154 us per force launch at N = 1M; the FP32 form,
+        // with less arithmetic per gather, lost 1% and does not)
+        if constexpr (PREC == 0) __builtin_amdgcn_s_setprio(VLJ_READ_PRIO);
 #pragma unroll
         for (int c = 0; c < 8; c++) {
           const uint32_t off = (w[c >> 1] >> ((c & 1) * 16)) & 0xffffu;  // 8 e
@@ -1349,6 +1357,7 @@ __device__ __forceinline__ PairAcc interactions_staged(const Buffers& B, const R
           dy[c] = *(const double*)((const char*)S.y + off) - yi;
           dz[c] = *(const double*)((const char*)S.z + off) - zi;
         }
+        if constexpr (PREC == 0) __builtin_amdgcn_s_setprio(0);
         if constexpr (PREC == 1 && !ENERGIES) {
           float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
