@@ -1991,43 +1991,66 @@ hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, 
   return hipGetLastError();
 }
 
-hipError_t launch_rebuild(const Buffers& B, hipStream_t s) {
+// The stream the sparse-layout main launch goes to: the side stream, forked
+// from s, when both layouts have bricks; s otherwise. join_side() makes s
+// wait for it again.
+static hipStream_t fork_side(const Buffers& B, hipStream_t s, const Side* side, hipError_t& e) {
+  if (!side || !side->stream || !layout_count<Dense>(B) || !layout_count<Sparse>(B)) return s;
+  if (e == hipSuccess) e = hipEventRecord(side->fork, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(side->stream, side->fork, 0);
+  return side->stream;
+}
+
+static void join_side(hipStream_t s, hipStream_t ss, const Side* side, hipError_t& e) {
+  if (ss == s) return;
+  if (e == hipSuccess) e = hipEventRecord(side->join, ss);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s, side->join, 0);
+}
+
+hipError_t launch_rebuild(const Buffers& B, hipStream_t s, const Side* side) {
   if (!B.n_atom_blocks) return hipSuccess;
   hipLaunchKernelGGL(k_bin, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_scan, grid1(B.n_scan_blocks), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_scatter, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_sort, grid1(dealt_grid(B.n_sort_blocks)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(k_reorder, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
-  if (B.n_bricks) VLJ_BRICK_LAUNCHES(k_build, B);
-  return hipGetLastError();
+  if (!B.n_bricks) return hipGetLastError();
+  hipError_t e = hipSuccess;
+  const hipStream_t ss = fork_side(B, s, side, e);
+  if (layout_count<Dense>(B))
+    hipLaunchKernelGGL((k_build<Dense, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B);
+  if (layout_count<Sparse>(B))
+    hipLaunchKernelGGL((k_build<Sparse, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, ss, B);
+  join_side(s, ss, side, e);
+  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_build<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, B);
+  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_build<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, B);
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
 template <int PREC, bool SF>
-void force_launches(const Buffers& B, int k, hipStream_t s) {
+hipError_t force_launches(const Buffers& B, int k, hipStream_t s, const Side* side) {
   const dim3 gp = parts_grid(B);
+  hipError_t e = hipSuccess;
+  const hipStream_t ss = fork_side(B, s, side, e);
   if (layout_count<Dense>(B))
     hipLaunchKernelGGL((k_force<Dense, PREC, SF, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B, k);
   if (layout_count<Sparse>(B))
-    hipLaunchKernelGGL((k_force<Sparse, PREC, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B, k);
+    hipLaunchKernelGGL((k_force<Sparse, PREC, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, ss, B, k);
+  join_side(s, ss, side, e);
   if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_force<Dense, PREC, SF, true>), gp, dim3(Dense::kThreads), 0, s, B, k);
   if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_force<Sparse, PREC, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B, k);
+  return e;
 }
 
-hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s) {
+hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s, const Side* side) {
   if (!B.n_atom_blocks) return hipSuccess;
   const int k = kick ? 1 : 0;
-  if (precision == 0) {
-    if (shifted)
-      force_launches<0, true>(B, k, s);
-    else
-      force_launches<0, false>(B, k, s);
-  } else {
-    if (shifted)
-      force_launches<1, true>(B, k, s);
-    else
-      force_launches<1, false>(B, k, s);
-  }
-  return hipGetLastError();
+  hipError_t e;
+  if (precision == 0)
+    e = shifted ? force_launches<0, true>(B, k, s, side) : force_launches<0, false>(B, k, s, side);
+  else
+    e = shifted ? force_launches<1, true>(B, k, s, side) : force_launches<1, false>(B, k, s, side);
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
 template <int PREC, bool SF>

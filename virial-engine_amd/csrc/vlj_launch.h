@@ -7,9 +7,18 @@
 
 namespace vlj {
 
+// A second stream for the sparse-layout brick launches, forked from and
+// joined back into the main stream by events. The two layouts' bricks hold
+// disjoint atoms, so their launches may overlap: the sparse bricks fill the
+// GPU while the dense launch's last workgroups run.
+struct Side {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
 hipError_t launch_decide(const Buffers& B, hipStream_t s);
 hipError_t launch_drift(const Buffers& B, bool langevin, hipStream_t s);
-hipError_t launch_rebuild(const Buffers& B, hipStream_t s);
+hipError_t launch_rebuild(const Buffers& B, hipStream_t s, const Side* side = nullptr);
 // Nose-Hoover half step: kinetic energy, xi / eta / scale; with `scale` the
 // velocities are scaled too (the second half step of a step).
 hipError_t launch_nh_half(const Buffers& B, bool scale, hipStream_t s);
@@ -17,7 +26,8 @@ hipError_t launch_nh_half(const Buffers& B, bool scale, hipStream_t s);
 hipError_t launch_heat_exchange(const Buffers& B, hipStream_t s);
 // One heat-current sample into sample `sample` of the series.
 hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, hipStream_t s);
-hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s);
+hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s,
+                        const Side* side = nullptr);
 hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStream_t s);
 hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s);
 // The float4 render arrays (positions; velocities with the drawn value in w).

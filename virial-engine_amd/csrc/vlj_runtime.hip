@@ -49,6 +49,7 @@ struct HostReplica {
 struct vlj_sim {
   int device = 0;
   hipStream_t stream = nullptr;
+  vlj::Side side;  // the sparse-layout brick launches, beside the dense ones
   std::string error = "no error";
   std::vector<Replica> rep;
   std::vector<HostReplica> host;
@@ -272,6 +273,11 @@ void free_all(vlj_sim* s) {
   s->allocations.clear();
   if (s->B.hc) (void)hipFree(s->B.hc);
   s->B.hc = nullptr;
+  if (s->side.stream) (void)hipStreamSynchronize(s->side.stream);
+  if (s->side.fork) (void)hipEventDestroy(s->side.fork);
+  if (s->side.join) (void)hipEventDestroy(s->side.join);
+  if (s->side.stream) (void)hipStreamDestroy(s->side.stream);
+  s->side = vlj::Side{};
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
 }
@@ -297,8 +303,8 @@ int check_device(vlj_sim* s) {
 int enqueue_step(vlj_sim* s) {
   if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, false, s->stream));
   HIP_TRY(s, launch_drift(s->B, s->any_langevin, s->stream));
-  HIP_TRY(s, launch_rebuild(s->B, s->stream));
-  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, true, s->stream));
+  HIP_TRY(s, launch_rebuild(s->B, s->stream, &s->side));
+  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, true, s->stream, &s->side));
   if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, true, s->stream));
   if (s->any_heat_exchange) HIP_TRY(s, launch_heat_exchange(s->B, s->stream));
   return VLJ_OK;
@@ -335,8 +341,8 @@ int capture(vlj_sim* s, uint32_t steps, hipGraphExec_t* out) {
 int prime_if_needed(vlj_sim* s) {
   if (s->primed) return VLJ_OK;
   HIP_TRY(s, launch_decide(s->B, s->stream));
-  HIP_TRY(s, launch_rebuild(s->B, s->stream));
-  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, false, s->stream));
+  HIP_TRY(s, launch_rebuild(s->B, s->stream, &s->side));
+  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, false, s->stream, &s->side));
   s->primed = true;
   return VLJ_OK;
 }
@@ -551,6 +557,15 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
     return fail(nullptr, VLJ_E_HIP, "hipStreamCreate failed");
+  }
+  if (s->n_dense > 0 && s->n_dense < s->bricks.size()) {  // both layouts: a side stream (default priority)
+    if (hipStreamCreateWithFlags(&s->side.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->side.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->side.join, hipEventDisableTiming) != hipSuccess) {
+      free_all(s);
+      delete s;
+      return fail(nullptr, VLJ_E_HIP, "hipStreamCreate failed");
+    }
   }
   Buffers& B = s->B;
   const size_t N = s->n_total, C = s->cells_total;
