@@ -289,10 +289,10 @@ int vlj_debug_params(const vlj_sim *sim, double out[6]);
 
 /* Workgroup-to-table mapping (no counterpart in the reference). With
  * blocks != 0: the table entry workgroup `block` works on -- workgroups go
- * round-robin to the 8 XCDs, and the table is dealt to them in chunks of 16
+ * round-robin to the 8 XCDs, and the table is dealt to them in chunks of 4
  * consecutive entries, chunk c to XCD c % 8. With blocks == 0: the grid
  * launched for a table of `block` entries (rounded up to whole rounds of
- * 128 workgroups; those past the end return at once). */
+ * 32 workgroups; those past the end return at once). */
 uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks);
 
 /* The force loop's reciprocal 1 / x for x in [1e-4, 1e37] (n values), for

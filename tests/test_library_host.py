@@ -132,16 +132,20 @@ def test_table_dealing_covers_every_entry_once(n):
     from virial_particles import _lib
 
     f = _lib.lib().vlj_debug_swizzle
+    # one round is 8 chunks; a one-entry table gets exactly one round
+    rnd = f(1, 0)
+    chunk = rnd // 8
+    assert rnd == 8 * chunk and chunk == 4
     grid = f(n, 0)
-    assert grid % 128 == 0 and n <= grid < n + 128
+    assert grid % rnd == 0 and n <= grid < n + rnd
     m = [f(b, 1) for b in range(grid)]
     assert sorted(m) == list(range(grid))
-    # chunks of 16 consecutive entries, chunk c on XCD c % 8 (= b % 8)
+    # chunks of `chunk` consecutive entries, chunk c on XCD c % 8 (= b % 8)
     for b, idx in enumerate(m):
-        assert (idx // 16) % 8 == b % 8
-    # a stretch of 128 consecutive entries touches all eight XCDs
-    if n >= 128:
-        assert {b % 8 for b, idx in enumerate(m) if idx < 128} == set(range(8))
+        assert (idx // chunk) % 8 == b % 8
+    # a stretch of one round of consecutive entries touches all eight XCDs
+    if n >= rnd:
+        assert {b % 8 for b, idx in enumerate(m) if idx < rnd} == set(range(8))
 
 
 def test_measurement_helpers(lj):
