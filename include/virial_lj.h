@@ -17,7 +17,13 @@
  *
  * One handle holds one or more independent periodic systems ("replicas").
  * vlj_create makes a handle with one; vlj_batch_create one with many, run by
- * the same kernels in the same launches over a replica-offset table.
+ * the same kernels in the same launches over a replica-offset table. The
+ * calls that act on one system (measure, poll, measure_now, box_size,
+ * target_temperature, set_temperature, the heat exchange and heat current,
+ * read_profile and read_profile_raw) return VLJ_E_STATE for a handle of
+ * several replicas, which has its own vlj_batch_* calls; on such a handle
+ * vlj_count is the total over the replicas and vlj_volume and vlj_time are
+ * NaN.
  */
 #ifndef VIRIAL_LJ_H
 #define VIRIAL_LJ_H
@@ -123,7 +129,7 @@ int vlj_measure_now(vlj_sim *sim, vlj_measurement *out);
 /* LennardJones::read_particles (rs:1354-1374): full precision, original order. */
 int vlj_read_particles(vlj_sim *sim, vlj_particle *out, size_t count);
 
-/* Accessors and setters (rs:870-1030). */
+/* Accessors and setters (rs:870-1030). count: particles of the handle. */
 uint32_t vlj_count(const vlj_sim *sim);
 int vlj_box_size(const vlj_sim *sim, double out[3]);
 double vlj_volume(const vlj_sim *sim);
@@ -190,6 +196,9 @@ const char *vlj_last_error(const vlj_sim *sim);
 int vlj_batch_create(const vlj_particle *all, const uint64_t *offsets,
                      const vlj_settings *settings, size_t replicas, vlj_batch **out);
 int vlj_batch_run(vlj_batch *batch, uint64_t steps);
+/* set_temperature (rs:905-925) of one replica: VLJ_E_STATE "set_temperature
+ * needs a thermostat" when it has none, VLJ_E_INVALID for no such replica. */
+int vlj_batch_set_temperature(vlj_batch *batch, size_t replica, double temperature);
 /* One measurement per replica, in replica order. */
 int vlj_batch_measure_now(vlj_batch *batch, vlj_measurement *per_replica, size_t replicas);
 
@@ -287,6 +296,17 @@ int vlj_debug_counters(const vlj_sim *sim, uint64_t out[6]);
  * force_shift, c1, c2. */
 int vlj_debug_params(const vlj_sim *sim, double out[6]);
 
+/* Overrides of the planning rules, for tests (no counterpart in the
+ * reference); each value 0 keeps its rule. list_capacity: list entries per
+ * atom (at least 8) in place of the capacity rule, so that lists overflow.
+ * stage_fill: dense bricks are planned to stage at most this share of the
+ * LDS staging limit (in place of 0.9; above 1 plans bricks that are cut
+ * into parts). sparse_fill: a z layer whose bricks stage at most this share
+ * of the sparse layout's limit takes that layout (in place of 0.9; negative:
+ * every layer dense). They apply to handles created (or loaded) afterwards,
+ * in every thread of the process. */
+int vlj_debug_plan(uint32_t list_capacity, double stage_fill, double sparse_fill);
+
 /* Workgroup-to-table mapping (no counterpart in the reference). With
  * blocks != 0: the table entry workgroup `block` works on -- workgroups go
  * round-robin to the 8 XCDs, and the table is dealt to them in chunks of 4
@@ -299,6 +319,11 @@ uint32_t vlj_debug_swizzle(uint32_t block, uint32_t blocks);
  * the check that it rounds as IEEE division does. No counterpart in the
  * reference. */
 int vlj_debug_rcp(const double *x, double *out, size_t n);
+/* The FP32 force loop's reciprocal 1 / x and square root for positive
+ * normal x (n values each), for the check that they round as IEEE division
+ * and square root do (R5f's 1.0f / q and sqrtf). No counterpart in the
+ * reference. */
+int vlj_debug_rcp32(const float *x, float *rcp, float *root, size_t n);
 /* Plain streaming copy of `bytes` (read once, written once) between two
  * device buffers, `repeats` times; *ms = average kernel time from HIP
  * events. Measures the bandwidth a streaming kernel reaches and calibrates

@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
         L.vo_log.restype = d
         L.vo_sincos_turns.argtypes = [d, dp, dp]
         L.vo_set_threads.argtypes = [i32]
+        L.vo_set_capacity_override.argtypes = [u32]
         L.vo_max_threads.restype = i32
         _lib = L
     return _lib
@@ -253,6 +254,20 @@ def sincos_turns(u: float):
     s, c = C.c_double(), C.c_double()
     lib().vo_sincos_turns(u, C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+class capacity_override:
+    """List capacity in place of the rule for sims created inside the block."""
+
+    def __init__(self, capacity: int):
+        self.capacity = capacity
+
+    def __enter__(self):
+        lib().vo_set_capacity_override(self.capacity)
+        return self
+
+    def __exit__(self, *exc):
+        lib().vo_set_capacity_override(0)
 
 
 def set_threads(n: int) -> None:

@@ -480,6 +480,10 @@ static double wrap_initial(double x, double side)
 }
 
 /* R2 */
+static uint32_t capacity_override; /* 0: the capacity rule */
+
+void vo_set_capacity_override(uint32_t capacity) { capacity_override = capacity; }
+
 static inline uint32_t cell_of_position(const vo_params *p, double x, double y, double z)
 {
     float q[3] = {(float)x, (float)y, (float)z};
@@ -553,13 +557,10 @@ int vo_derive_params(const double *pos, size_t n, const double box_in[3], double
     double local = (double)occ_max / cell_volume;
     double use = local > density ? local : density;
     p->capacity = (uint32_t)ceil(2.0 * sphere * use + 32.0);
-    /* VLJ_LIST_CAPACITY, when set, replaces the rule (at least 8): the tests'
+    /* vo_set_capacity_override may replace the rule (at least 8): the tests'
      * way to reach list overflow, which the rule avoids for the initial state */
-    const char *cap_env = getenv("VLJ_LIST_CAPACITY");
-    if (cap_env) {
-        int c = atoi(cap_env);
-        p->capacity = (uint32_t)(c > 8 ? c : 8);
-    }
+    if (capacity_override)
+        p->capacity = capacity_override > 8 ? capacity_override : 8;
 
     double kinetic = 0.0, speed = 0.0;
     for (size_t i = 0; i < n; i++) {

@@ -137,6 +137,10 @@ void vo_langevin_noise(uint64_t seed, uint32_t stream, uint32_t atom, uint64_t s
 double vo_log(double x);
 void vo_sincos_turns(double u, double *s, double *c);
 
+/* List capacity in place of the rule for sims created afterwards (0: the
+ * rule; at least 8), so that tests reach list overflow. */
+void vo_set_capacity_override(uint32_t capacity);
+
 void vo_set_threads(int threads);
 int vo_max_threads(void);
 

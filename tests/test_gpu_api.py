@@ -167,18 +167,17 @@ def test_dense_cluster_takes_the_global_memory_path(oracle, lj):
 
 
 @pytest.mark.parametrize("layout", ["dense", "sparse"])
-def test_bricks_cut_into_parts_stay_exact(oracle, lj, monkeypatch, layout):
+def test_bricks_cut_into_parts_stay_exact(oracle, lj, layout):
     # bricks planned far beyond what LDS holds (8 x 8 home cells of a
     # liquid stage ~5700 atoms): each is cut into runs of home rows (dense
     # layout, 3200 atoms) or of home cells (sparse layout, 1024); lists,
     # forces, measurements, profiles and the heat current stay exact
-    monkeypatch.setenv("VLJ_STAGE_FILL", "100")
-    monkeypatch.setenv("VLJ_SPARSE_FILL", "100" if layout == "sparse" else "0")
     size = [24.0, 24.0, 12.0]
     count = int(round(0.8442 * np.prod(size)))
     pos, vel = oracle.lattice_box(count, size, 1.0, 5)
     st = lj.Settings(size, 2.5, 0.002, shifted_force=True, profile_bins=6)
-    sim = lj.LennardJones((pos, vel), st)
+    with lj.debug_plan(stage_fill=100.0, sparse_fill=100.0 if layout == "sparse" else -1.0):
+        sim = lj.LennardJones((pos, vel), st)
     ref = oracle.Sim(pos, vel, size, dt=0.002, shifted_force=True)
     ref.set_profile_bins(6)
     assert sim.debug_grid()["cells"] == [8, 8, 4]
@@ -256,19 +255,16 @@ def test_fp64_fma_rate_is_plausible(lj):
     assert 20.0 < tf < 1.2 * 78.6
 
 
-def test_fp32_form_is_the_same_on_cut_bricks(oracle, lj, monkeypatch):
+def test_fp32_form_is_the_same_on_cut_bricks(oracle, lj):
     # the FP32 form sums each list octet in FP32 by list position, so bricks
     # cut into parts (other staging, same list order) give the same bits
     size = [24.0, 24.0, 12.0]
     count = int(round(0.8442 * np.prod(size)))
     pos, vel = oracle.lattice_box(count, size, 1.0, 5)
     runs = []
-    for fill in (None, "100"):
-        if fill is None:
-            monkeypatch.delenv("VLJ_STAGE_FILL", raising=False)
-        else:
-            monkeypatch.setenv("VLJ_STAGE_FILL", fill)
-        sim = lj.LennardJones((pos, vel), lj.Settings(size, 2.5, 0.002, precision=1))
+    for fill in (0.0, 100.0):
+        with lj.debug_plan(stage_fill=fill):
+            sim = lj.LennardJones((pos, vel), lj.Settings(size, 2.5, 0.002, precision=1))
         sim.run(40)
         runs.append((list(sim.measure_now().raw), sim.debug_state()[0], sim.debug_counters()))
     assert runs[0][2]["stage_overflow_builds"] == 0 and runs[1][2]["stage_overflow_builds"] > 0

@@ -2,8 +2,9 @@
 
 Integer work (cells, cell order, lists, fixed-point sums) must be bit-exact.
 FP64 forces, energies and trajectories follow the same IEEE operation
-sequence as the oracle, so they are compared bit for bit as well; the FP32
-pair-force form is compared with the FP64 oracle within stated tolerances.
+sequence as the oracle, so they are compared bit for bit as well. The FP32
+pair-force form follows the oracle's R5f rule bit for bit too, and is
+compared with the FP64 oracle within stated tolerances.
 """
 import numpy as np
 import pytest
@@ -72,6 +73,46 @@ def test_force_reciprocal_rounds_like_division(lj):
     assert bad.size == 0, f"{bad.size} differ, first x = {x[bad[:5]]!r}"
 
 
+def test_fp32_reciprocal_and_root_round_like_ieee(lj):
+    # R5f's 1.0f / q and sqrtf(q): every float of two binades (the rules
+    # scale by powers of two) and the range q = (float)r^2 takes
+    rng = np.random.default_rng(11)
+    binades = [((np.uint32(e) << np.uint32(23)) | np.arange(1 << 23, dtype=np.uint32)).view(np.float32)
+               for e in (126, 127)]
+    x = np.concatenate(binades + [
+        (10.0 ** rng.uniform(-4, 36, 2_000_000)).astype(np.float32),
+        np.float32(2.0) ** np.arange(-13, 120, dtype=np.float32),
+        np.array([3e36, 6.25, 12.25, 7.84, 1e-4], dtype=np.float32),
+    ])
+    rcp, root = lj.debug_rcp32(x)
+    want = np.float32(1.0) / x
+    bad = np.flatnonzero(rcp.view(np.uint32) != want.view(np.uint32))
+    assert bad.size == 0, f"reciprocal: {bad.size} differ, first x = {x[bad[:5]]!r}"
+    want = np.sqrt(x)
+    bad = np.flatnonzero(root.view(np.uint32) != want.view(np.uint32))
+    assert bad.size == 0, f"square root: {bad.size} differ, first x = {x[bad[:5]]!r}"
+
+
+@pytest.mark.parametrize("shifted_force", [False, True])
+def test_fp32_form_bit_exact_with_the_oracle(oracle, lj, shifted_force):
+    # R5f on both sides: the FP32 pair forces, summed in FP32 per list
+    # octet, and the FP32 pair energies give the oracle's bits
+    pos, vel, box = sc_liquid(oracle, n_side=12)
+    pos = oracle.jitter(pos, 99, 0.3)
+    sim, ref = make_pair(oracle, lj, pos, vel, box, precision=1, shifted_force=shifted_force)
+    for steps in (0, 7, 60):
+        sim.run(steps)
+        ref.run(steps)
+        m, r = sim.measure_now(), ref.measure()
+        assert list(m.raw) == list(r.raw), f"after {sim.steps()} steps"
+    p, v, a, o = sim.debug_state()
+    rp, rv, ra, ro = ref.state()
+    np.testing.assert_array_equal(o, ro)
+    np.testing.assert_array_equal(a, ra)
+    np.testing.assert_array_equal(p, rp)
+    np.testing.assert_array_equal(v, rv)
+
+
 def test_cell_grid_finds_every_pair(oracle, lj):
     # tests/thermodynamics.rs:55-98, at the reference's tolerance and bit for bit
     pos, vel = jittered_lattice(oracle)
@@ -108,10 +149,11 @@ def test_rectangular_box_matches_all_pairs(oracle, lj, shifted_force):
 
 
 def test_nve_trajectory_bit_exact(oracle, lj):
-    # SC 10^3 at rho 0.8442, T 0.722, rc 2.5 truncated-shifted, dt 0.005
+    # SC 10^3 at rho 0.8442, T 0.722, rc 2.5 truncated-shifted, dt 0.005:
+    # the SC-1000 NVE configuration over its full 1000 steps
     pos, vel, box = sc_liquid(oracle)
     sim, ref = make_pair(oracle, lj, pos, vel, box)
-    for steps in (1, 9, 90):
+    for steps in (1, 9, 90, 900):
         sim.run(steps)
         ref.run(steps)
         m, r = sim.measure_now(), ref.measure()
@@ -163,13 +205,35 @@ def test_dilute_lists_match_the_oracle(oracle, lj, rho):
     assert list(sim.measure_now().raw) == list(ref.measure().raw)
 
 
-def test_list_overflow_matches_the_oracle(oracle, lj, monkeypatch):
+def test_neighbourhood_past_8192_atoms_keeps_exact_lists(oracle, lj):
+    # cutoff 7 at rho 0.85: 3 cells per axis, so every one-cell brick's 27
+    # neighbour cells hold all 9261 atoms, beyond LDS and beyond what a
+    # list entry of 8 x the staged index can hold in 16 bits; the global
+    # path stores the staged index itself and the lists stay exact
+    n = 21 ** 3
+    box = (n / 0.85) ** (1.0 / 3.0)
+    pos, vel = oracle.lattice(n, box, 0.722, 4)
+    pos = oracle.jitter(pos, 7, 0.2)
+    sim, ref = make_pair(oracle, lj, pos, vel, box, cutoff=7.0, dt=0.002)
+    assert sim.debug_grid()["cells"] == [3, 3, 3]
+    for steps in (0, 20):
+        sim.run(steps)
+        ref.run(steps)
+        m, r = sim.measure_now(), ref.measure()
+        assert m.list_overflow == r.list_overflow == 0
+        assert list(m.raw) == list(r.raw)
+    assert sim.debug_counters()["global_cell_builds"] > 0
+    assert_same_grid(sim, ref)
+    np.testing.assert_array_equal(sim.debug_state()[2], ref.state()[2])
+
+
+def test_list_overflow_matches_the_oracle(oracle, lj):
     # lennard_jones.wgsl:494-497: past the capacity a list keeps its first
     # `capacity` entries and the measurement reports the longest list found;
     # a capacity of 40 (the rule gives ~190 here) overflows every list
-    monkeypatch.setenv("VLJ_LIST_CAPACITY", "40")
     pos, vel = jittered_lattice(oracle)
-    sim, ref = make_pair(oracle, lj, pos, vel, 11.0, dt=0.002)
+    with lj.debug_plan(list_capacity=40), oracle.capacity_override(40):
+        sim, ref = make_pair(oracle, lj, pos, vel, 11.0, dt=0.002)
     for steps in (0, 50):
         sim.run(steps)
         ref.run(steps)

@@ -87,19 +87,18 @@ def test_resume_heat_exchange_profiles_and_heat_current(oracle, lj, tmp_path):
     same_state(sim, back)
 
 
-def test_resume_batch_with_cut_bricks(oracle, lj, tmp_path, monkeypatch):
-    monkeypatch.setenv("VLJ_STAGE_FILL", "100")  # bricks that are cut into parts
+def test_resume_batch_with_cut_bricks(oracle, lj, tmp_path):
     pos_a, vel_a, box_a = sc_liquid(oracle, n_side=14)
     pos_b, vel_b, box_b = sc_liquid(oracle, n_side=9)
     specs = [((pos_a, vel_a), lj.Settings([box_a] * 3, 2.5, 0.005, thermostat=lj.Langevin(0.9, 1.0, 5))),
              ((pos_b, vel_b), lj.Settings([box_b] * 3, 2.5, 0.005))]
-    batch = lj.Batch(specs)
+    with lj.debug_plan(stage_fill=100.0):  # bricks that are cut into parts
+        batch = lj.Batch(specs)
     batch.run(70)
     assert batch.debug_counters()["stage_overflow_builds"] > 0
     path = tmp_path / "batch.state"
     batch.save(path)
-    monkeypatch.setenv("VLJ_STAGE_FILL", "0.9")  # the loaded batch keeps the saved bricks
-    back = lj.Batch.load(path)
+    back = lj.Batch.load(path)  # planned with the rules, it keeps the saved bricks
     assert back.sizes == batch.sizes and back.replicas == 2
     assert back.debug_counters()["bricks"] == batch.debug_counters()["bricks"]
     batch.run(150)

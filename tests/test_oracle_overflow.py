@@ -6,13 +6,13 @@ import numpy as np
 from helpers import jittered_lattice
 
 
-def test_overflowing_lists_keep_their_first_entries(oracle, monkeypatch):
+def test_overflowing_lists_keep_their_first_entries(oracle):
     pos, vel = jittered_lattice(oracle)
     full = oracle.Sim(pos, vel, [11.0] * 3)
     assert full.measure().list_overflow == 0
     fc, fe = full.lists()
-    monkeypatch.setenv("VLJ_LIST_CAPACITY", "40")
-    cut = oracle.Sim(pos, vel, [11.0] * 3)
+    with oracle.capacity_override(40):
+        cut = oracle.Sim(pos, vel, [11.0] * 3)
     m = cut.measure()
     assert cut.params.capacity == 40
     cc, ce = cut.lists()

@@ -219,6 +219,17 @@ constexpr uint32_t kEntryShift = 3;
 static_assert((kStageMax + 1u) << kEntryShift <= 0x10000u, "list entries are 16 bits");
 __host__ __device__ __forceinline__ uint32_t entry_of(uint32_t e) { return e << kEntryShift; }
 __host__ __device__ __forceinline__ uint32_t staged_of(uint32_t entry) { return entry >> kEntryShift; }
+// A brick (or part) whose neighbourhood does not fit in LDS reads positions
+// through the cell tables instead, and its list entries are the staged
+// indices e themselves, so that neighbourhoods of up to 65535 atoms keep
+// exact lists. Every kernel that reads a list knows which case it is in
+// (the same staged total, from the same cell counts).
+__host__ __device__ __forceinline__ uint32_t list_index(uint32_t entry, bool staged) {
+  return staged ? staged_of(entry) : entry;
+}
+__host__ __device__ __forceinline__ uint32_t list_entry_of(uint32_t e, bool staged) {
+  return staged ? entry_of(e) : e;
+}
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr double kFar = 1.0e18;  // sentinel coordinate: r^2 ~ 3e36, beyond any cutoff
 

@@ -1022,11 +1022,12 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
     B.yb[i] = yi;
     B.zb[i] = zi;
     // A neighbourhood too large for 16-bit staged indices keeps no list;
-    // the overflow is reported (lennard_jones.wgsl:494-497).
+    // the overflow is reported whatever the capacity, with the staged count
+    // as the length (lennard_jones.wgsl:494-497).
     if (total > 0xffffu) B.nbr_count[i] = 0u;
   }
   if (total > 0xffffu) {
-    if (threadIdx.x == 0 && H > 0u && total > cap) atomicMax(&B.ctl[g.rep].overflow, total);
+    if (threadIdx.x == 0 && H > 0u) atomicMax(&B.ctl[g.rep].overflow, total);
     return;
   }
   // Work items: the atoms of each home cell cut into pieces of at most
@@ -1099,7 +1100,7 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
       const uint32_t nc = (T - c0 + 63u) / 64u;  // chunks of this group that hold candidates
       uint32_t qe[kGroup];
 #pragma unroll
-      for (int c = 0; c < kGroup; c++) qe[c] = entry_of(q[c]);
+      for (int c = 0; c < kGroup; c++) qe[c] = list_entry_of(q[c], staged);
       for (uint32_t k = k0; k < k1; k++) {
         const uint32_t e = ebase + k;
         double xi, yi, zi;
@@ -1220,6 +1221,23 @@ __device__ __forceinline__ double rcp_rn(double x) {
   return __builtin_fma(e, y, y);
 }
 
+// 1 / q correctly rounded in FP32, for a positive normal q whose reciprocal
+// is normal (R5f's 1.0f / q): the hardware estimate (within one ulp), one
+// Newton-Raphson correction y + y (1 - q y) with both steps fused (1 - q y
+// is exact), which rounds to the nearest float for every faithful estimate
+// except one case: a significand of all ones, whose reciprocal 2^k (1 +
+// 2^-24 + 2^-48 + ...) lies just above a midpoint, so that the estimate
+// 2^k gives the tie 2^k (1 + 2^-24) and rounds back down to 2^k. There the
+// answer is 2^k (1 + 2^-23) from either estimate, i.e. the last
+// significand bit set.
+__device__ __forceinline__ float rcp_rn_f32(float q) {
+  const float y0 = __builtin_amdgcn_rcpf(q);
+  const float e = __builtin_fmaf(-q, y0, 1.0f);
+  const float y1 = __builtin_fmaf(y0, e, y0);
+  const uint32_t m = __float_as_uint(q) & 0x7fffffu;
+  return __uint_as_float(__float_as_uint(y1) | ((m + 1u) >> 23));
+}
+
 // R5 (FP64) and R5f (FP32 pair arithmetic on FP64 displacements), without
 // branches: the terms of a pair outside the cutoff are computed and then
 // replaced by zero, which leaves every sum bit for bit as if the pair had
@@ -1269,7 +1287,7 @@ __device__ __forceinline__ void pair_accumulate(const Replica& R, double dx, dou
   } else {
     static_assert(PREC == 0 || ENERGIES, "FP32 forces sum through pair_force_f32");
     const float q = (float)r2;
-    const float inv2 = __builtin_amdgcn_rcpf(q);
+    const float inv2 = rcp_rn_f32(q);
     const float s6 = inv2 * inv2 * inv2;
     {
       float rf = s6 * __builtin_fmaf(48.0f, s6, -24.0f);
@@ -1299,7 +1317,7 @@ __device__ __forceinline__ void pair_force_f32(const Replica& R, double dx, doub
                                                float& fx, float& fy, float& fz) {
   const bool hit = r2 < R.rc2;
   const float q = (float)r2;
-  const float inv2 = __builtin_amdgcn_rcpf(q);
+  const float inv2 = rcp_rn_f32(q);
   const float s6 = inv2 * inv2 * inv2;
   float g = s6 * __builtin_fmaf(s6, 2.0f, -1.0f);
   if (SF) g = __builtin_fmaf(-__builtin_sqrtf(q), (float)R.force_shift24, g);
@@ -1390,7 +1408,7 @@ __device__ __forceinline__ PairAcc interactions_global(const Buffers& B, const R
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;  // FP32 forces: the current octet's sum
   for (uint32_t k = 0; k < cnt; k++) {
     double xj, yj, zj;
-    staged_pos(R, S, false, staged_of(lst[list_entry(k)]), B.x, B.y, B.z, xj, yj, zj);
+    staged_pos(R, S, false, list_index(lst[list_entry(k)], false), B.x, B.y, B.z, xj, yj, zj);
     const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
     if constexpr (PREC == 1 && !ENERGIES) {
       pair_force_f32<SF>(R, dx, dy, dz, dist2(dx, dy, dz), fx, fy, fz);
@@ -1601,7 +1619,7 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
     const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
     for (uint32_t k = 0; k < cnt; k++) {
       double xj, yj, zj;
-      staged_pos(R, S, staged, staged_of(lst[list_entry(k)]), B.x, B.y, B.z, xj, yj, zj);
+      staged_pos(R, S, staged, list_index(lst[list_entry(k)], staged), B.x, B.y, B.z, xj, yj, zj);
       const double d[3] = {xj - xi, yj - yi, zj - zi};
       const double r2 = dist2(d[0], d[1], d[2]);
       if (!(r2 < R.rc2)) continue;
@@ -1721,7 +1739,7 @@ __device__ __forceinline__ void heat_part(const Buffers& B, const Replica& R, co
     const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
     for (uint32_t k = 0; k < cnt; k++) {
       double xj, yj, zj;
-      staged_pos(R, S, staged, staged_of(lst[list_entry(k)]), B.x, B.y, B.z, xj, yj, zj);
+      staged_pos(R, S, staged, list_index(lst[list_entry(k)], staged), B.x, B.y, B.z, xj, yj, zj);
       const double d[3] = {xj - xi, yj - yi, zj - zi};
       const double r2 = dist2(d[0], d[1], d[2]);
       if (!(r2 < R.rc2)) continue;
@@ -1797,7 +1815,7 @@ __device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, 
     const uint32_t cnt = B.nbr_count[i];
     const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
     for (uint32_t k = 0; k < cnt; k++) {
-      const uint32_t e = staged_of(lst[list_entry(k)]);
+      const uint32_t e = list_index(lst[list_entry(k)], staged);
       double xj, yj, zj;
       staged_pos(R, S, staged, e, B.x, B.y, B.z, xj, yj, zj);
       const double r2 = dist2(xj - xi, yj - yi, zj - zi);
@@ -1861,6 +1879,16 @@ __global__ void __launch_bounds__(kBlock) k_fixed_sum(const double* values, size
 __global__ void __launch_bounds__(kBlock) k_rcp(const double* x, double* out, size_t n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) out[t] = rcp_rn(x[t]);
+}
+
+// rcp_rn_f32 and the FP32 square root of the shifted-force pair over an
+// array (checks of their rounding against IEEE division and square root).
+__global__ void __launch_bounds__(kBlock) k_rcp32(const float* x, float* rcp, float* root, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    rcp[t] = rcp_rn_f32(x[t]);
+    root[t] = __builtin_sqrtf(x[t]);
+  }
 }
 
 __global__ void k_zero_i64(long long* p, uint32_t n) {
@@ -2104,6 +2132,11 @@ hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s) {
 
 hipError_t launch_rcp(const double* x, double* out, size_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_rcp, grid1((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, x, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_rcp32(const float* x, float* rcp, float* root, size_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_rcp32, grid1((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, x, rcp, root, n);
   return hipGetLastError();
 }
 

@@ -34,6 +34,7 @@ hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s);
 hipError_t launch_render(const Buffers& B, hipStream_t s);
 hipError_t launch_zero(long long* p, uint32_t n, hipStream_t s);
 hipError_t launch_rcp(const double* x, double* out, size_t n, hipStream_t s);
+hipError_t launch_rcp32(const float* x, float* rcp, float* root, size_t n, hipStream_t s);
 hipError_t launch_fixed_sum(const double* values, size_t n, double scale, long long* out,
                             hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t words, hipStream_t s);

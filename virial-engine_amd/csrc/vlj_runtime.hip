@@ -28,6 +28,15 @@ constexpr double kPi = 3.14159265358979323846;  // the f64 nearest pi (std::f64:
 
 thread_local std::string g_error = "no error";
 
+// Overrides of the planning rules for tests (vlj_debug_plan); zero keeps a
+// rule. Read when a handle is created.
+struct PlanOverrides {
+  uint32_t list_capacity = 0;  // list entries per atom (at least 8)
+  double stage_fill = 0.0;     // dense-layout bricks stage at most this share of kStageMax
+  double sparse_fill = 0.0;    // a layer takes the sparse layout under this share of its stage max
+};
+PlanOverrides g_plan;
+
 struct Ring {
   long long* slots = nullptr;  // pinned
   Control* ctl = nullptr;      // pinned
@@ -81,6 +90,21 @@ int fail(vlj_sim* s, int code, const std::string& msg) {
   g_error = msg;
   return code;
 }
+
+// Calls that act on one system refuse a handle of several replicas, which
+// has its own vlj_batch_* calls.
+int single_only(vlj_sim* s, const char* what) {
+  if (s->rep.size() <= 1) return VLJ_OK;
+  return fail(s, VLJ_E_STATE,
+              std::string(what) + " acts on one system; this handle holds " + std::to_string(s->rep.size()) +
+                  " replicas (use the vlj_batch_* calls)");
+}
+
+#define SINGLE_ONLY(sim, what)                  \
+  do {                                          \
+    int rc_ = single_only((sim), (what));       \
+    if (rc_ != VLJ_OK) return rc_;              \
+  } while (0)
 
 #define HIP_TRY(sim, expr)                                                             \
   do {                                                                                 \
@@ -184,10 +208,10 @@ int derive(const vlj_particle* p, size_t n, const vlj_settings& st, Replica& R, 
   const double cell_volume = cell_size[0] * cell_size[1] * cell_size[2];
   const double use = std::max((double)occ_max / cell_volume, density);
   R.capacity = (uint32_t)std::ceil(2.0 * sphere * use + 32.0);
-  // VLJ_LIST_CAPACITY, when set, replaces the rule (at least 8): tests use a
-  // small capacity to reach list overflow, which the rule itself avoids for
-  // the initial state
-  if (const char* c = std::getenv("VLJ_LIST_CAPACITY")) R.capacity = (uint32_t)std::max(8, std::atoi(c));
+  // vlj_debug_plan may replace the rule (at least 8): tests use a small
+  // capacity to reach list overflow, which the rule itself avoids for the
+  // initial state
+  if (g_plan.list_capacity) R.capacity = std::max(8u, g_plan.list_capacity);
 
   double kinetic = 0.0, speed = 0.0;
   for (size_t i = 0; i < n; i++) {
@@ -357,9 +381,9 @@ int prime_if_needed(vlj_sim* s) {
 // layers' counts over the periodically extended plane. A brick that
 // outgrows LDS later on is cut into parts on the device. A layer whose
 // bricks are held to at most 90% of the sparse layout's 1024 atoms takes
-// that layout. The environment variables VLJ_STAGE_FILL and VLJ_SPARSE_FILL,
-// when set, replace the two 90% (a value above 1 plans bricks that will be
-// cut, 0 keeps every layer dense: for tests and tuning).
+// that layout. vlj_debug_plan may replace the two 90% (a value above 1 plans
+// bricks that will be cut, a negative sparse fill keeps every layer dense:
+// for tests and tuning).
 void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vector<uint32_t>& bx,
                    std::vector<uint32_t>& by, std::vector<uint8_t>& sparse) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
@@ -370,8 +394,8 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
 #define VLJ_MEAN_FLOOR 1.0
 #endif
   double fill = VLJ_STAGE_FILL, sparse_fill = VLJ_STAGE_FILL;
-  if (const char* f = std::getenv("VLJ_STAGE_FILL")) fill = std::atof(f);
-  if (const char* f = std::getenv("VLJ_SPARSE_FILL")) sparse_fill = std::atof(f);
+  if (g_plan.stage_fill != 0.0) fill = g_plan.stage_fill;
+  if (g_plan.sparse_fill != 0.0) sparse_fill = g_plan.sparse_fill;
   const double limit = fill * kStageMax;
   const double sparse_limit = sparse_fill * Sparse::kStageMax;
   const double mean = VLJ_MEAN_FLOOR * (double)R.count / (double)R.total_cells;
@@ -786,6 +810,7 @@ int vlj_run(vlj_sim* sim, uint64_t steps) {
 
 int vlj_measure(vlj_sim* sim, int32_t* accepted) {
   if (!sim || !accepted) return fail(sim, VLJ_E_INVALID, "sim and accepted must not be NULL");
+  SINGLE_ONLY(sim, "measure");
   int slot = free_slot(sim);
   if (slot < 0) {
     *accepted = 0;
@@ -798,6 +823,7 @@ int vlj_measure(vlj_sim* sim, int32_t* accepted) {
 
 int vlj_poll(vlj_sim* sim, vlj_measurement* out, size_t capacity, size_t* written) {
   if (!sim || !written || (capacity && !out)) return fail(sim, VLJ_E_INVALID, "bad arguments to vlj_poll");
+  SINGLE_ONLY(sim, "poll");
   *written = 0;
   std::vector<std::pair<uint64_t, int>> done;
   for (int k = 0; k < kRing; k++) {
@@ -821,6 +847,7 @@ int vlj_poll(vlj_sim* sim, vlj_measurement* out, size_t capacity, size_t* writte
 
 int vlj_measure_now(vlj_sim* sim, vlj_measurement* out) {
   if (!sim || !out) return fail(sim, VLJ_E_INVALID, "sim and out must not be NULL");
+  SINGLE_ONLY(sim, "measure_now");
   int slot = free_slot(sim);
   if (slot < 0) return fail(sim, VLJ_E_STATE, "a measurement is already in flight");
   int rc = enqueue_measure(sim, slot);
@@ -880,37 +907,45 @@ int vlj_read_particles(vlj_sim* sim, vlj_particle* out, size_t count) {
   return vlj_batch_read_particles(sim, out, count);
 }
 
-uint32_t vlj_count(const vlj_sim* sim) { return sim ? sim->rep[0].count : 0u; }
+uint32_t vlj_count(const vlj_sim* sim) { return sim ? sim->n_total : 0u; }
 
 int vlj_box_size(const vlj_sim* sim, double out[3]) {
   if (!sim || !out) return VLJ_E_INVALID;
+  SINGLE_ONLY(const_cast<vlj_sim*>(sim), "box_size");
   for (int k = 0; k < 3; k++) out[k] = sim->rep[0].box[k];
   return VLJ_OK;
 }
 
 double vlj_volume(const vlj_sim* sim) {
-  return sim ? sim->rep[0].box[0] * sim->rep[0].box[1] * sim->rep[0].box[2] : 0.0;
+  if (!sim) return 0.0;
+  if (sim->rep.size() > 1) return std::nan("");
+  return sim->rep[0].box[0] * sim->rep[0].box[1] * sim->rep[0].box[2];
 }
 
 uint64_t vlj_steps(const vlj_sim* sim) { return sim ? sim->steps : 0u; }
 
-double vlj_time(const vlj_sim* sim) { return sim ? (double)sim->steps * sim->rep[0].dt : 0.0; }
+double vlj_time(const vlj_sim* sim) {
+  if (!sim) return 0.0;
+  if (sim->rep.size() > 1) return std::nan("");
+  return (double)sim->steps * sim->rep[0].dt;
+}
 
 int vlj_target_temperature(const vlj_sim* sim, double* temperature, int32_t* has) {
   if (!sim || !temperature || !has) return VLJ_E_INVALID;
+  SINGLE_ONLY(const_cast<vlj_sim*>(sim), "target_temperature");
   const vlj_thermostat& t = sim->host[0].settings.thermostat;
   *has = t.kind != VLJ_THERMOSTAT_NONE ? 1 : 0;
   *temperature = *has ? t.temperature : 0.0;
   return VLJ_OK;
 }
 
-int vlj_set_temperature(vlj_sim* sim, double temperature) {
-  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
-  vlj_thermostat& t = sim->host[0].settings.thermostat;
+static int set_temperature(vlj_sim* sim, size_t replica, double temperature) {
+  if (replica >= sim->rep.size()) return fail(sim, VLJ_E_INVALID, "no such replica");
+  vlj_thermostat& t = sim->host[replica].settings.thermostat;
   if (t.kind == VLJ_THERMOSTAT_NONE) return fail(sim, VLJ_E_STATE, "set_temperature needs a thermostat");
   if (!(temperature >= 0.0)) return fail(sim, VLJ_E_INVALID, "temperature must be non-negative");
   t.temperature = temperature;
-  Replica& R = sim->rep[0];
+  Replica& R = sim->rep[replica];
   if (t.kind == VLJ_THERMOSTAT_LANGEVIN) {
     const double c1 = std::exp(-t.friction * R.dt);
     R.c2 = std::sqrt((1.0 - c1 * c1) * temperature);
@@ -920,9 +955,21 @@ int vlj_set_temperature(vlj_sim* sim, double temperature) {
     R.nh_gkt = dof * temperature;
     R.nh_q = R.nh_gkt * R.nh_period * R.nh_period / (4.0 * (kPi * kPi));
   }
-  HIP_TRY(sim, hipMemcpyAsync((void*)&sim->B.rep[0], &R, sizeof(Replica), hipMemcpyHostToDevice, sim->stream));
+  HIP_TRY(sim, hipMemcpyAsync((void*)&sim->B.rep[replica], &R, sizeof(Replica), hipMemcpyHostToDevice,
+                              sim->stream));
   HIP_TRY(sim, hipStreamSynchronize(sim->stream));
   return VLJ_OK;
+}
+
+int vlj_set_temperature(vlj_sim* sim, double temperature) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  SINGLE_ONLY(sim, "set_temperature");
+  return set_temperature(sim, 0, temperature);
+}
+
+int vlj_batch_set_temperature(vlj_batch* batch, size_t replica, double temperature) {
+  if (!batch) return fail(nullptr, VLJ_E_INVALID, "batch must not be NULL");
+  return set_temperature(batch, replica, temperature);
 }
 
 int vlj_set_condensed_neighbours(vlj_sim* sim, uint32_t neighbours) {
@@ -980,6 +1027,8 @@ static int copy_raw(vlj_sim* sim, const std::vector<long long>& raw, int64_t* ou
 // lennard_jones.rs:1297-1325: the sums decoded from fixed point and divided
 // by the sample count (NaN with no samples, as 0 / 0 there).
 int vlj_read_profile(vlj_sim* sim, double* out, size_t capacity, uint64_t* samples) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  SINGLE_ONLY(sim, "read_profile");
   std::vector<long long> raw;
   uint64_t n = 0;
   int rc = read_profile_sums(sim, 0, raw, &n);
@@ -992,6 +1041,8 @@ int vlj_read_profile(vlj_sim* sim, double* out, size_t capacity, uint64_t* sampl
 }
 
 int vlj_read_profile_raw(vlj_sim* sim, int64_t* raw_out, size_t capacity, uint64_t* samples) {
+  if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  SINGLE_ONLY(sim, "read_profile_raw");
   std::vector<long long> raw;
   int rc = read_profile_sums(sim, 0, raw, samples);
   return rc != VLJ_OK ? rc : copy_raw(sim, raw, raw_out, capacity);
@@ -1008,6 +1059,7 @@ int vlj_batch_read_profile_raw(vlj_batch* sim, size_t replica, int64_t* raw_out,
 // or loses the exchange kernels.
 int vlj_set_heat_exchange(vlj_sim* sim, int32_t on, const double hot[2], const double cold[2], double energy) {
   if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  SINGLE_ONLY(sim, "set_heat_exchange");
   Replica& R = sim->rep[0];
   if (R.thermostat != 0u)
     return fail(sim, VLJ_E_STATE, "heat exchange runs at constant energy, without a thermostat");
@@ -1033,6 +1085,7 @@ int vlj_set_heat_exchange(vlj_sim* sim, int32_t on, const double hot[2], const d
 // lennard_jones.rs:945-951: energy moved per step over 2 A dt.
 int vlj_heat_flux(const vlj_sim* sim, double* flux, int32_t* has) {
   if (!sim || !flux || !has) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_heat_flux");
+  SINGLE_ONLY(const_cast<vlj_sim*>(sim), "heat_flux");
   const Replica& R = sim->rep[0];
   *has = R.hex_on ? 1 : 0;
   *flux = R.hex_on ? R.hex_energy / (2.0 * R.box[0] * R.box[1] * R.dt) : 0.0;
@@ -1042,6 +1095,7 @@ int vlj_heat_flux(const vlj_sim* sim, double* flux, int32_t* has) {
 // lennard_jones.rs:953-973: room for `capacity` samples, any recorded dropped.
 int vlj_record_heat_current(vlj_sim* sim, uint32_t capacity) {
   if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  SINGLE_ONLY(sim, "record_heat_current");
   HIP_TRY(sim, hipStreamSynchronize(sim->stream));
   if (sim->B.hc) HIP_TRY(sim, hipFree(sim->B.hc));
   sim->B.hc = nullptr;
@@ -1062,6 +1116,7 @@ int vlj_record_heat_current(vlj_sim* sim, uint32_t capacity) {
 // lennard_jones.rs:975-996: one sample, asynchronous.
 int vlj_sample_heat_current(vlj_sim* sim) {
   if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  SINGLE_ONLY(sim, "sample_heat_current");
   if (!sim->B.hc) return fail(sim, VLJ_E_STATE, "record_heat_current first");
   if (sim->hc_samples >= sim->rep[0].hc_cap) return fail(sim, VLJ_E_STATE, "heat current series full");
   int rc = prime_if_needed(sim);
@@ -1073,6 +1128,7 @@ int vlj_sample_heat_current(vlj_sim* sim) {
 
 static int read_series(vlj_sim* sim, std::vector<long long>& raw, uint32_t* samples) {
   if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
+  SINGLE_ONLY(sim, "read_heat_current");
   if (!sim->B.hc) return fail(sim, VLJ_E_STATE, "record_heat_current first");
   raw.assign((size_t)sim->hc_samples * 3, 0);
   if (!raw.empty()) {
@@ -1197,7 +1253,8 @@ int vlj_debug_lists(vlj_sim* sim, uint32_t* counts, uint32_t* entries) {
   if (rc != VLJ_OK) return rc;
   if (counts) std::memcpy(counts, cnt.data(), cnt.size() * 4);
   if (entries) {
-    // Entries are (8 x) staged indices of the atom's brick; rebuild each brick's
+    // Entries are 8 x the staged indices of the atom's brick (the indices
+    // themselves where the brick reads global memory); rebuild each brick's
     // staging order from the cell tables to turn them into slots.
     const size_t padded = (size_t)((R.count + kListStride - 1) / kListStride) * kListStride;
     std::vector<uint16_t> t((size_t)list_octets(R.capacity) * 8u * padded);
@@ -1227,11 +1284,12 @@ int vlj_debug_lists(vlj_sim* sim, uint32_t* counts, uint32_t* entries) {
             run += ncell[cell];
           }
       off.push_back(run);
+      const bool staged = run <= (uint32_t)(sim->host[0].sparse[cz] ? Sparse::kStageMax : Dense::kStageMax);
       const uint64_t base = list_base(R, i) - R.list_off;
       for (uint32_t k = 0; k < R.capacity; k++) {
         uint32_t slot = 0;
         if (k < cnt[i]) {
-          const uint32_t e = staged_of(t[base + list_entry(k)]);
+          const uint32_t e = list_index(t[base + list_entry(k)], staged);
           size_t sc = std::upper_bound(off.begin(), off.end(), e) - off.begin() - 1;
           while (sc + 1 < off.size() && off[sc + 1] <= e) sc++;
           slot = sorted[first[sc] + (e - off[sc])] - R.atom_off;
@@ -1282,6 +1340,15 @@ int vlj_debug_params(const vlj_sim* sim, double out[6]) {
   out[3] = sim->rep[0].force_shift;
   out[4] = sim->rep[0].c1;
   out[5] = sim->rep[0].c2;
+  return VLJ_OK;
+}
+
+int vlj_debug_plan(uint32_t list_capacity, double stage_fill, double sparse_fill) {
+  if (!std::isfinite(stage_fill) || !std::isfinite(sparse_fill) || stage_fill < 0.0)
+    return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_debug_plan");
+  g_plan.list_capacity = list_capacity;
+  g_plan.stage_fill = stage_fill;
+  g_plan.sparse_fill = sparse_fill;
   return VLJ_OK;
 }
 
@@ -1367,6 +1434,28 @@ int vlj_debug_rcp(const double* x, double* out, size_t n) {
   if (dx) (void)hipFree(dx);
   if (dout) (void)hipFree(dout);
   if (e != hipSuccess) return fail(nullptr, VLJ_E_HIP, std::string("vlj_debug_rcp: ") + hipGetErrorString(e));
+  return VLJ_OK;
+}
+
+int vlj_debug_rcp32(const float* x, float* rcp, float* root, size_t n) {
+  if (n && (!x || !rcp || !root)) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_debug_rcp32");
+  vlj_sim tmp;
+  int rc = check_device(&tmp);
+  if (rc != VLJ_OK) return rc;
+  float *dx = nullptr, *dr = nullptr, *ds = nullptr;
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(float);
+  hipError_t e = hipMalloc(&dx, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dr, bytes);
+  if (e == hipSuccess) e = hipMalloc(&ds, bytes);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_rcp32(dx, dr, ds, n, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(rcp, dr, n * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(root, ds, n * sizeof(float), hipMemcpyDeviceToHost);
+  if (dx) (void)hipFree(dx);
+  if (dr) (void)hipFree(dr);
+  if (ds) (void)hipFree(ds);
+  if (e != hipSuccess) return fail(nullptr, VLJ_E_HIP, std::string("vlj_debug_rcp32: ") + hipGetErrorString(e));
   return VLJ_OK;
 }
 

@@ -10,8 +10,7 @@ import ctypes as C
 import os
 
 PACKAGE_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-# VLJ_LIBRARY selects another build of the same library (a tuning variant).
-LIB_PATH = os.environ.get("VLJ_LIBRARY", os.path.join(PACKAGE_ROOT, "libvirial_lj.so"))
+LIB_PATH = os.path.join(PACKAGE_ROOT, "libvirial_lj.so")
 
 OK = 0
 E_INVALID = 1
@@ -118,6 +117,7 @@ SIGNATURES = {
     "vlj_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "vlj_replica_info": (C.c_int, [_P, _sz, C.POINTER(Settings), _u64p, C.POINTER(_sz)]),
     "vlj_batch_run": (C.c_int, [_P, _u64]),
+    "vlj_batch_set_temperature": (C.c_int, [_P, _sz, _d]),
     "vlj_batch_measure_now": (C.c_int, [_P, C.POINTER(Measurement), _sz]),
     "vlj_batch_replicas": (_sz, [_P]),
     "vlj_batch_read_particles": (C.c_int, [_P, _pp, _sz]),
@@ -133,10 +133,12 @@ SIGNATURES = {
     "vlj_debug_grid": (C.c_int, [_P, _up]),
     "vlj_debug_params": (C.c_int, [_P, _dp]),
     "vlj_debug_counters": (C.c_int, [_P, _u64p]),
+    "vlj_debug_plan": (C.c_int, [_u32, _d, _d]),
     "vlj_debug_swizzle": (_u32, [_u32, _u32]),
     "vlj_copy_bandwidth": (C.c_int, [_sz, _u32, _dp]),
     "vlj_fp64_fma_rate": (C.c_int, [_u32, _dp]),
     "vlj_debug_rcp": (C.c_int, [_dp, _dp, _sz]),
+    "vlj_debug_rcp32": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), _sz]),
     "vlj_timed_run": (C.c_int, [_P, _u64, _dp, _dp]),
 }
 

@@ -671,6 +671,10 @@ class Batch:
     def run(self, steps: int) -> None:
         _check(L.lib().vlj_batch_run(self._h, steps), self._h)
 
+    def set_temperature(self, replica: int, temperature: float) -> None:
+        """set_temperature (lennard_jones.rs:905-925) of one replica."""
+        _check(L.lib().vlj_batch_set_temperature(self._h, replica, temperature), self._h)
+
     def measure_now(self) -> List[Measurement]:
         out = (L.Measurement * self.replicas)()
         _check(L.lib().vlj_batch_measure_now(self._h, out, self.replicas), self._h)
@@ -743,12 +747,40 @@ def random_insertion_array(count: int, box, min_distance: float, temperature: fl
     return out
 
 
+class debug_plan:
+    """Overrides the planning rules for handles created inside the block
+    (tests only): list_capacity in place of the capacity rule, stage_fill and
+    sparse_fill in place of the brick planner's 0.9 (see vlj_debug_plan)."""
+
+    def __init__(self, list_capacity: int = 0, stage_fill: float = 0.0, sparse_fill: float = 0.0):
+        self.values = (list_capacity, stage_fill, sparse_fill)
+
+    def __enter__(self):
+        _check(L.lib().vlj_debug_plan(*self.values))
+        return self
+
+    def __exit__(self, *exc):
+        _check(L.lib().vlj_debug_plan(0, 0.0, 0.0))
+
+
 def debug_rcp(x) -> np.ndarray:
     """The force loop's reciprocal of each x, computed on the device."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     out = np.zeros_like(x)
     _check(L.lib().vlj_debug_rcp(_dptr(x), _dptr(out), x.shape[0]))
     return out
+
+
+def debug_rcp32(x):
+    """The FP32 force loop's reciprocal and square root of each x (float32),
+    computed on the device."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    rcp = np.zeros_like(x)
+    root = np.zeros_like(x)
+    fp = C.POINTER(C.c_float)
+    _check(L.lib().vlj_debug_rcp32(x.ctypes.data_as(fp), rcp.ctypes.data_as(fp), root.ctypes.data_as(fp),
+                                   x.shape[0]))
+    return rcp, root
 
 
 def copy_bandwidth(nbytes: int, repeats: int = 10) -> float:
