@@ -273,6 +273,20 @@ int vlj_random_insertion(size_t count, const double box[3], double min_distance,
  * 64-bit atomic per workgroup (fixed.wgsl:71-105; KAT tests/gpu.rs:89-149). */
 int vlj_fixed_sum(const double *values, size_t count, double fixed_scale, int64_t *out);
 
+/* ---- fixed-point sum without allocation (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. The same
+ * sum as vlj_fixed_sum, but `values` (count doubles) and `out` (one int64)
+ * are device pointers the caller owns, `stream` is a hipStream_t (NULL: the
+ * default stream), and the call only enqueues: *out is written when the
+ * stream gets there. It allocates nothing: its partial sums (one per
+ * workgroup, at most 1024) and a ticket live in a device array of about
+ * 8.5 KB inside the library, one copy per GPU, held until the process ends.
+ * At most 2^36 values (VLJ_E_INVALID beyond). Calls that overlap on one GPU
+ * (from other threads or streams) race on that array. */
+int vlj_fixed_sum_device(const double *values, size_t count, double fixed_scale, int64_t *out,
+                         void *stream);
+
 /* ---- inspection, for parity tests (no counterpart in the reference) ----
  * State in slot order (the device keeps particles in cell order). Each
  * pointer may be NULL. */

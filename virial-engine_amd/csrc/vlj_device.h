@@ -80,6 +80,8 @@ using Dense = Layout<kBrickThreads, kStageMax, false>;
 using Sparse = Layout<128, VLJ_SPARSE_STAGE_MAX, true>;
 constexpr int kListStride = 64;   // lists are transposed per block of 64 atoms (one wave)
 constexpr int kSlots = 8;         // fixed-point slots per replica
+constexpr uint32_t kFixedSumBlocks = 1024;  // workgroups of the no-allocation fixed-point sum, at most
+constexpr uint64_t kFixedSumMaxCount = 1ull << 36;  // values one no-allocation sum takes, at most
 constexpr double kNearR2 = 2.25;  // condensed: neighbours within 1.5 sigma (lennard_jones.wgsl:417)
 
 // Parameters of one replica (lennard_jones.rs:282-313, :443-560).

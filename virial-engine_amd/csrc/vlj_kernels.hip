@@ -12,6 +12,8 @@
 //   build       brick-staged Verlet lists                     } (grid.wgsl, wgsl:419-499)
 //   force       brick-staged pair forces + second half kick (wgsl:501-574)
 // The sequence is static, so it is captured once into a hipGraph.
+#include <mutex>
+
 #include "vlj_device.h"
 #include "vlj_launch.h"
 
@@ -2152,6 +2154,91 @@ hipError_t launch_fixed_sum(const double* values, size_t n, double scale, long l
     const uint32_t blocks = (uint32_t)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_fixed_sum, grid1(blocks), dim3(kBlock), 0, s, values, n, scale, out);
   }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The fixed-point sum without allocation (vlj_fixed_sum_device): its
+// per-workgroup partial sums and last-workgroup ticket live in a file-scope
+// device array, one copy per GPU, whose address is looked up once per device.
+// Every call on a GPU shares that GPU's copy: calls that overlap on one GPU
+// race. The ticket is zero when the module loads and the last workgroup of
+// every call sets it back to zero, so no call needs to clear anything.
+// ---------------------------------------------------------------------------
+
+struct FixedSumScratch {
+  long long partial[kFixedSumBlocks];  // one per workgroup of the launch
+  uint32_t ticket;                     // workgroups done
+  uint32_t pad[63];
+};
+static __device__ __align__(256) FixedSumScratch scratch;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+FixedSumScratch* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up on the
+// first call there). In host code the name `scratch` is only a handle for
+// the lookup, never a device address.
+static hipError_t scratch_of_device(FixedSumScratch** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch[dev] = (FixedSumScratch*)p;
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
+
+// Grid-stride sum of round-ties-even(values[i] * scale) per workgroup into
+// sc->partial; the last workgroup adds the partials and writes *out.
+__global__ void __launch_bounds__(kBlock) k_fixed_sum_partials(const double* values, size_t n, double scale,
+                                                                 FixedSumScratch* sc, long long* out) {
+  __shared__ long long part[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  long long acc = 0;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    acc += to_fixed(values[i], scale);
+  acc = wave_sum_i64(acc);
+  if (lane == 0) part[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) t += part[w];
+    sc->partial[blockIdx.x] = t;
+  }
+  if (!last_block(&sc->ticket)) return;
+  long long t = 0;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock)
+    t += __hip_atomic_load(&sc->partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  t = wave_sum_i64(t);
+  __syncthreads();  // part[] was read above by thread 0
+  if (lane == 0) part[wave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) total += part[w];
+    *out = total;
+    atomicExch(&sc->ticket, 0u);
+  }
+}
+
+hipError_t launch_fixed_sum_noalloc(const double* values, size_t n, double scale, long long* out, hipStream_t s) {
+  FixedSumScratch* sc = nullptr;
+  hipError_t e = scratch_of_device(&sc);
+  if (e != hipSuccess) return e;
+  const size_t want = (n + kBlock - 1) / kBlock;
+  const uint32_t blocks = (uint32_t)std::max<size_t>(1, std::min<size_t>(want, kFixedSumBlocks));
+  hipLaunchKernelGGL(k_fixed_sum_partials, grid1(blocks), dim3(kBlock), 0, s, values, n, scale, sc, out);
   return hipGetLastError();
 }
 

@@ -37,6 +37,11 @@ hipError_t launch_rcp(const double* x, double* out, size_t n, hipStream_t s);
 hipError_t launch_rcp32(const float* x, float* rcp, float* root, size_t n, hipStream_t s);
 hipError_t launch_fixed_sum(const double* values, size_t n, double scale, long long* out,
                             hipStream_t s);
+// The same sum with no allocation: partial sums in a file-scope device
+// array of kFixedSumBlocks entries (looked up once per device); *out is
+// device memory. Calls that overlap on one GPU race.
+hipError_t launch_fixed_sum_noalloc(const double* values, size_t n, double scale, long long* out,
+                                    hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t words, hipStream_t s);
 hipError_t launch_fma(double* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 

@@ -1210,6 +1210,15 @@ int vlj_fixed_sum(const double* values, size_t count, double fixed_scale, int64_
   return VLJ_OK;
 }
 
+int vlj_fixed_sum_device(const double* values, size_t count, double fixed_scale, int64_t* out, void* stream) {
+  if (!out || (count && !values)) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_fixed_sum_device");
+  if ((uint64_t)count > kFixedSumMaxCount)
+    return fail(nullptr, VLJ_E_INVALID, "vlj_fixed_sum_device takes at most 2^36 values");
+  hipError_t e = launch_fixed_sum_noalloc(values, count, fixed_scale, (long long*)out, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(nullptr, VLJ_E_HIP, std::string("vlj_fixed_sum_device: ") + hipGetErrorString(e));
+  return VLJ_OK;
+}
+
 int vlj_debug_state(vlj_sim* sim, double* pos, double* vel, double* acc, uint32_t* orig) {
   if (!sim) return fail(nullptr, VLJ_E_INVALID, "sim must not be NULL");
   std::vector<double> p, v, a;

@@ -127,6 +127,7 @@ SIGNATURES = {
     "vlj_fcc_count": (C.c_int, [_sz, _d, _d, _u64, _pp, _dp]),
     "vlj_random_insertion": (C.c_int, [_sz, _dp, _d, _d, _u64, _pp]),
     "vlj_fixed_sum": (C.c_int, [_dp, _sz, _d, _i64p]),
+    "vlj_fixed_sum_device": (C.c_int, [_P, _sz, _d, _P, _P]),
     "vlj_debug_state": (C.c_int, [_P, _dp, _dp, _dp, _up]),
     "vlj_debug_cells": (C.c_int, [_P, _up, _up, _up]),
     "vlj_debug_lists": (C.c_int, [_P, _up, _up]),
