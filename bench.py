@@ -81,33 +81,6 @@ def run_form(lj, particles, side, precision, steps, warmup, timed_steps):
     }
 
 
-def batch_workload(lj, replicas, seed):
-    """The batched-replica workload: N log-uniform in [500, 50000], rho
-    uniform in [0.01, 0.85], T uniform in [0.7, 2.0]; FCC sites for rho >= 0.3,
-    random insertion 0.9 apart below; Langevin gamma 1 with a seed per
-    replica; rc 2.5, skin 0.3, dt 0.005, FP64."""
-    import math
-
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    specs, meta = [], []
-    for _ in range(replicas):
-        n = int(round(math.exp(rng.uniform(math.log(500.0), math.log(50000.0)))))
-        rho = float(rng.uniform(0.01, 0.85))
-        temp = float(rng.uniform(0.7, 2.0))
-        rseed = int(rng.integers(1, 1 << 62))
-        if rho >= 0.3:
-            parts, side = lj.fcc_count_array(n, rho, temp, rseed)
-        else:
-            side = float(np.float32((n / rho) ** (1.0 / 3.0)))
-            parts = lj.random_insertion_array(n, [side] * 3, 0.9, temp, rseed)
-        st = lj.Settings([side] * 3, 2.5, 0.005, skin=0.3, thermostat=lj.Langevin(temp, 1.0, rseed))
-        specs.append((parts, st))
-        meta.append({"n": n, "rho": n / side ** 3, "T": temp})
-    return specs, meta
-
-
 def run_batch(lj, specs, meta, steps, warmup, block_steps, blocks):
     """Runs the batch; times the stepping only. Pressure is sampled every
     block_steps steps and PV/NkT gets a standard error from `blocks` block
@@ -217,39 +190,18 @@ def precision_study(lj, cells, steps, every, equilibrate=1000):
     return out
 
 
-def slab_replica(lj, cells, seed):
-    """The liquid-vapour slab: an FCC liquid block of cells^3 unit cells at
-    rho ~0.73 spanning the middle quarter of a box with Lz = 4 Lx, random
-    vapour at rho 0.02 (0.9 apart, 1 sigma clear of the liquid) elsewhere;
-    T 0.7 velocities."""
-    import numpy as np
-
-    liquid, lx = lj.fcc_array(cells, 0.73, 0.7, seed)
-    lz = 4.0 * lx
-    liquid["pos"][:, 2] += 1.5 * lx  # the block fills z in [1.5 Lx, 2.5 Lx)
-    gap = 1.0
-    height = 3.0 * lx - 2.0 * gap
-    n_vap = int(round(0.02 * lx * lx * height))
-    vapour = lj.random_insertion_array(n_vap, [lx, lx, height], 0.9, 0.7, seed + 1)
-    vapour["pos"][:, 2] = np.mod(vapour["pos"][:, 2] + 2.5 * lx + gap, lz)
-    parts = np.concatenate([liquid, vapour])
-    vel = parts["vel"] - parts["vel"].mean(axis=0)
-    parts["vel"] = vel
-    return parts, [lx, lx, lz]
-
-
 def run_slab(lj, cells, seeds, steps, warmup, sample_every, bins):
     """8 seeds of the slab in one batch (force-shifted rc 2.5, Langevin gamma
     0.5 at T 0.7, FP32 pair forces, `bins` z slabs); times the stepping and
     samples the profiles every `sample_every` steps."""
     import numpy as np
 
+    from virial_particles import workloads as W
+
     specs = []
     for k in range(seeds):
-        parts, box = slab_replica(lj, cells, 100 + 2 * k)
-        st = lj.Settings(box, 2.5, 0.005, skin=0.3, shifted_force=True, profile_bins=bins, precision=1,
-                         thermostat=lj.Langevin(0.7, 0.5, 1000 + k))
-        specs.append((parts, st))
+        parts, box = W.slab_replica(cells, 100 + 2 * k)
+        specs.append((parts, W.slab_settings(box, 1000 + k, bins)))
     atoms = sum(p.shape[0] for p, _ in specs)
     b = lj.Batch(specs)
     b.run(warmup)
@@ -440,6 +392,7 @@ def main():
         sys.exit(2)
 
     from virial_particles import lennard_jones as lj
+    from virial_particles import workloads as W
 
     if args.slab:
         print(json.dumps(run_slab(lj, args.slab_cells, 8, args.steps, args.warmup, 100, 200)), flush=True)
@@ -448,7 +401,7 @@ def main():
         print(json.dumps(precision_study(lj, args.cells, args.steps, args.measure_every)), flush=True)
         return
     if args.batch_only:
-        specs, meta = batch_workload(lj, args.batch_replicas, 7)
+        specs, meta = W.batch_workload(args.batch_replicas, 7)
         br = run_batch(lj, specs, meta, args.batch_steps, args.warmup, 25, 20)
         br.pop("pv_nkt")
         print(json.dumps(br), flush=True)
@@ -503,7 +456,7 @@ def main():
     }
     batched = None
     if args.batch_replicas > 0:
-        specs, meta = batch_workload(lj, args.batch_replicas, 7)
+        specs, meta = W.batch_workload(args.batch_replicas, 7)
         br = run_batch(lj, specs, meta, args.batch_steps, args.warmup, 25, 20)
         zs = br.pop("pv_nkt")
         if args.batch_table:

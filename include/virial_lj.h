@@ -306,6 +306,9 @@ int vlj_debug_grid(const vlj_sim *sim, uint32_t out[6]);
  * read global memory (not even one cell's neighbourhood fits), and the
  * number of bricks with the sparse layout. No counterpart in the reference. */
 int vlj_debug_counters(const vlj_sim *sim, uint64_t out[6]);
+/* The same for one replica: list rebuilds, brick list builds that cut the
+ * brick into parts, list overflow, global-memory single-cell builds. */
+int vlj_debug_replica_counters(const vlj_sim *sim, size_t replica, uint64_t out[4]);
 /* Derived reals of replica 0: energy_scale, momentum_scale, shift,
  * force_shift, c1, c2. */
 int vlj_debug_params(const vlj_sim *sim, double out[6]);

@@ -1341,6 +1341,17 @@ int vlj_debug_counters(const vlj_sim* sim, uint64_t out[6]) {
   return VLJ_OK;
 }
 
+int vlj_debug_replica_counters(const vlj_sim* sim, size_t replica, uint64_t out[4]) {
+  if (!sim || !out || replica >= sim->rep.size()) return VLJ_E_INVALID;
+  Control c;
+  if (hipMemcpy(&c, sim->B.ctl + replica, sizeof(Control), hipMemcpyDeviceToHost) != hipSuccess) return VLJ_E_HIP;
+  out[0] = c.rebuilds;
+  out[1] = c.stage_overflow;
+  out[2] = c.overflow;
+  out[3] = c.stage_global;
+  return VLJ_OK;
+}
+
 int vlj_debug_params(const vlj_sim* sim, double out[6]) {
   if (!sim || !out) return VLJ_E_INVALID;
   out[0] = sim->host[0].energy_scale;

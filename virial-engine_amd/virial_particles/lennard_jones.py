@@ -691,6 +691,14 @@ class Batch:
     def debug_counters(self) -> dict:
         return _debug_counters(self._h)
 
+    def debug_replica_counters(self, replica: int) -> dict:
+        """One replica's list rebuilds, cut brick builds, list overflow and
+        global-memory single-cell builds."""
+        out = (C.c_uint64 * 4)()
+        _check(L.lib().vlj_debug_replica_counters(self._h, replica, out), self._h)
+        return {"rebuilds": out[0], "stage_overflow_builds": out[1], "list_overflow": out[2],
+                "global_cell_builds": out[3]}
+
     def read_profile_raw(self, replica: int):
         out = np.zeros(self.profile_bins[replica] * PROFILE_SUMS + 3, dtype=np.int64)
         samples = C.c_uint64(0)
