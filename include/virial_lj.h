@@ -243,6 +243,41 @@ size_t vlj_batch_replicas(const vlj_batch *batch);
 /* Particles of every replica, concatenated as given to vlj_batch_create. */
 int vlj_batch_read_particles(vlj_batch *batch, vlj_particle *out, size_t count);
 
+/* ---- brick planner and its tuner (no counterpart in the reference) ----
+ * The kernels stage each brick's neighbourhood (gx x gy home cells of one z
+ * layer and the cells around them) in LDS. Brick shapes are planned per
+ * replica and z layer from the cell occupancy, the largest whose every
+ * brick stages at most `fill` of the LDS staging limit (a window's count
+ * plus four of its Poisson standard deviations); the layout (workgroup
+ * size) follows from the staged count. Plans never change
+ * results: lists are built in the same order whatever the bricks.
+ *
+ * vlj_tune measures: for each candidate fill (0.8, 0.9, 1.0) it plans
+ * every replica's bricks, rebuilds the lists from the positions of their
+ * last build for that plan, runs `steps_per_candidate` real steps timing
+ * the pair-force launches with HIP events, and keeps the plan with the
+ * least force time plus list-build time per step. It blocks; *steps_taken
+ * gets the steps it ran (they count as steps of the run). A handle of at
+ * least 65,536 atoms tunes itself (6 steps per candidate) on the first
+ * vlj_step or vlj_run that asks for at least 64 steps. Every 512 steps the
+ * control words are copied back without a wait; a replica whose bricks
+ * were cut into parts at more than 2% of its bricks per list build is
+ * planned again from the occupancy of its last list build (its density
+ * rose since), with its lists rebuilt for the new bricks. VLJ_E_STATE when
+ * vlj_debug_plan fixes the plan. */
+typedef struct {
+    double fill;     /* the candidate's fill */
+    double force_us; /* mean pair-force time per step */
+    double build_us; /* list build over the whole brick table */
+    uint32_t bricks; /* bricks in the table */
+} vlj_tune_trial;
+int vlj_tune(vlj_sim *sim, uint32_t steps_per_candidate, uint64_t *steps_taken);
+/* The planner's state: the fill of the plan in use, the re-plans since the
+ * handle was created (tuning aside), and the trials of the last tuning
+ * (*n_trials of them; up to `capacity` written). */
+int vlj_plan_info(const vlj_sim *sim, double *fill, uint32_t *replans, vlj_tune_trial *trials,
+                  size_t capacity, size_t *n_trials);
+
 /* ---- lattices (rs:1377-1441), host only ---- */
 /* lattice: `count` particles on a simple cubic lattice filling a cube. */
 int vlj_lattice(size_t count, double box_size, double temperature, uint64_t seed,

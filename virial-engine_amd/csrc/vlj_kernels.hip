@@ -161,7 +161,10 @@ __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
     moved = (unsigned long long)__double_as_longlong(dist2(d[0], d[1], d[2]));
   }
   // One value per workgroup, no atomics (lennard_jones.wgsl:215-220 uses
-  // one global atomicMax per workgroup); k_decide reduces them.
+  // one global atomicMax per workgroup); k_decide reduces them. (A ticket
+  // per workgroup, so that the replica's last workgroup could decide, is
+  // itself an atomic on one address per replica: from the 4096 workgroups
+  // of a 1M-atom replica it made a step 130 us longer.)
   __shared__ unsigned long long wmax[kBlock / 64];
   moved = wave_max_u64(moved);
   if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = moved;
@@ -993,13 +996,15 @@ static_assert(kListStride == 64, "entry pos of a list lies (pos / 8) KB + 2 (pos
 // On a reorder the workgroup also copies its home atoms' state back from
 // the second set.
 // The lists of the home atoms of brick (or part) g, staged in S.
+// With `relist` the positions are those of the last build (xb) and the
+// state is left as it is: the same lists again, for another brick plan.
 template <class L>
 __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
-                                           bool staged, bool moved, const double* X, const double* Y,
+                                           bool staged, bool moved, bool relist, const double* X, const double* Y,
                                            const double* Z, uint32_t& next_item) {
   const uint32_t cap = R.capacity;
   const uint32_t total = S.total;
-  if (!staged && threadIdx.x == 0) atomicAdd(&B.ctl[g.rep].stage_global, 1u);
+  if (!staged && !relist && threadIdx.x == 0) atomicAdd(&B.ctl[g.rep].stage_global, 1u);
   const uint32_t nh = g.gx * g.gy;
   const uint32_t H = S.hoff[nh];
   for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
@@ -1020,9 +1025,11 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
       B.orig[i] = B.orig2[i];
       B.cell_of[i] = B.cell_of2[i];
     }
-    B.xb[i] = xi;
-    B.yb[i] = yi;
-    B.zb[i] = zi;
+    if (!relist) {
+      B.xb[i] = xi;
+      B.yb[i] = yi;
+      B.zb[i] = zi;
+    }
     // A neighbourhood too large for 16-bit staged indices keeps no list;
     // the overflow is reported whatever the capacity, with the staged count
     // as the length (lennard_jones.wgsl:494-497).
@@ -1148,22 +1155,24 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
 // marked to be cut and builds them part by part (PARTS true; launched after
 // the first, on the same step). A brick counts as cut when its
 // neighbourhood does not fit in LDS and it has more than one home cell.
+// `relist`: every brick of every replica again, from the positions of the
+// last build, after the brick plan changed; nothing else changes.
 template <class L, bool PARTS>
-__global__ void __launch_bounds__(L::kThreads) k_build(Buffers B) {
+__global__ void __launch_bounds__(L::kThreads) k_build(Buffers B, int relist) {
   __shared__ Stage<L> S;
   __shared__ uint32_t next_item;
-  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
+  if (!relist && B.flags[0] == 0u) return;  // no replica rebuilds this step
   if constexpr (!PARTS) {
     const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
     if (gb >= layout_end<L>(B)) return;
     const Brick g = B.bricks[gb];
     const Control& ctl = B.ctl[g.rep];
-    if (!ctl.rebuild) return;
-    const bool moved = ctl.reorder != 0u;
+    if (!relist && !ctl.rebuild) return;
+    const bool moved = !relist && ctl.reorder != 0u;
     const Replica R = B.rep[g.rep];
-    const double* X = moved ? B.x2 : B.x;
-    const double* Y = moved ? B.y2 : B.y;
-    const double* Z = moved ? B.z2 : B.z;
+    const double* X = relist ? B.xb : (moved ? B.x2 : B.x);
+    const double* Y = relist ? B.yb : (moved ? B.y2 : B.y);
+    const double* Z = relist ? B.zb : (moved ? B.z2 : B.z);
     if (threadIdx.x == 0) next_item = 0u;
     const bool staged = stage_brick(B, R, g, X, Y, Z, S);  // ends with a barrier
     const bool cut = !staged && g.gx * g.gy > 1u;
@@ -1175,25 +1184,25 @@ __global__ void __launch_bounds__(L::kThreads) k_build(Buffers B) {
         atomicAdd(&B.cut_groups[gb / kCutGroup], np != 0u ? 1u : 0xffffffffu);
         atomicAdd(&B.flags[1], np != 0u ? 1u : 0xffffffffu);
       }
-      if (cut) atomicAdd(&B.ctl[g.rep].stage_overflow, 1u);
+      if (cut && !relist) atomicAdd(&B.ctl[g.rep].stage_overflow, 1u);
     }
     if (cut) return;
-    build_part(B, R, S, g, staged, moved, X, Y, Z, next_item);
+    build_part(B, R, S, g, staged, moved, relist != 0, X, Y, Z, next_item);
   } else {
     each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
       const Brick brick = B.bricks[gb];
       const Control& ctl = B.ctl[brick.rep];
-      if (!ctl.rebuild) return;
-      const bool moved = ctl.reorder != 0u;
+      if (!relist && !ctl.rebuild) return;
+      const bool moved = !relist && ctl.reorder != 0u;
       const Replica R = B.rep[brick.rep];
-      const double* X = moved ? B.x2 : B.x;
-      const double* Y = moved ? B.y2 : B.y;
-      const double* Z = moved ? B.z2 : B.z;
+      const double* X = relist ? B.xb : (moved ? B.x2 : B.x);
+      const double* Y = relist ? B.yb : (moved ? B.y2 : B.y);
+      const double* Z = relist ? B.zb : (moved ? B.z2 : B.z);
       const Brick g = part_brick(brick, code, R);
       __syncthreads();  // the previous part is done with S and next_item
       if (threadIdx.x == 0) next_item = 0u;
       const bool staged = stage_brick(B, R, g, X, Y, Z, S);
-      build_part(B, R, S, g, staged, moved, X, Y, Z, next_item);
+      build_part(B, R, S, g, staged, moved, relist != 0, X, Y, Z, next_item);
     });
   }
 }
@@ -2048,13 +2057,24 @@ hipError_t launch_rebuild(const Buffers& B, hipStream_t s, const Side* side) {
   hipError_t e = hipSuccess;
   const hipStream_t ss = fork_side(B, s, side, e);
   if (layout_count<Dense>(B))
-    hipLaunchKernelGGL((k_build<Dense, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B);
+    hipLaunchKernelGGL((k_build<Dense, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B, 0);
   if (layout_count<Sparse>(B))
-    hipLaunchKernelGGL((k_build<Sparse, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, ss, B);
+    hipLaunchKernelGGL((k_build<Sparse, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, ss, B, 0);
   join_side(s, ss, side, e);
-  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_build<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, B);
-  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_build<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, B);
+  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_build<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, B, 0);
+  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_build<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, B, 0);
   return e != hipSuccess ? e : hipGetLastError();
+}
+
+hipError_t launch_relist(const Buffers& B, hipStream_t s) {
+  if (!B.n_bricks) return hipSuccess;
+  if (layout_count<Dense>(B))
+    hipLaunchKernelGGL((k_build<Dense, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B, 1);
+  if (layout_count<Sparse>(B))
+    hipLaunchKernelGGL((k_build<Sparse, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B, 1);
+  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_build<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, B, 1);
+  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_build<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, B, 1);
+  return hipGetLastError();
 }
 
 template <int PREC, bool SF>

@@ -16,9 +16,21 @@ struct Side {
   hipEvent_t fork = nullptr, join = nullptr;
 };
 
+// One candidate the brick planner's tuner measured: the fill its bricks were
+// planned with, the mean pair-force time per step and the time of the list
+// build over the whole table, from HIP events on the handle's stream.
+struct TuneTrial {
+  double fill, force_us, build_us;
+  uint32_t bricks;
+};
+
 hipError_t launch_decide(const Buffers& B, hipStream_t s);
 hipError_t launch_drift(const Buffers& B, bool langevin, hipStream_t s);
 hipError_t launch_rebuild(const Buffers& B, hipStream_t s, const Side* side = nullptr);
+// The lists of every replica again from the positions of the last build
+// (xb), for a new brick table; the cut marks of the table must start at
+// zero. Leaves the state as it is.
+hipError_t launch_relist(const Buffers& B, hipStream_t s);
 // Nose-Hoover half step: kinetic energy, xi / eta / scale; with `scale` the
 // velocities are scaled too (the second half step of a step).
 hipError_t launch_nh_half(const Buffers& B, bool scale, hipStream_t s);

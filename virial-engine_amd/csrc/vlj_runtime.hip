@@ -23,6 +23,21 @@ namespace {
 
 constexpr int kRing = 3;         // read-back slots (lennard_jones.rs:320)
 constexpr uint32_t kGraphSteps = 16;
+// Brick planner. Bricks are planned to stage at most kDefaultFill of the
+// LDS staging limit until the tuner has measured the candidates kTuneFills
+// (kTuneSteps steps each) on the handle's own state; a handle of at least
+// kTuneMinAtoms atoms tunes itself on the first call that asks for at least
+// kTuneAutoSteps steps. Every kCheckSteps steps the control words are copied
+// back without a wait, and a replica whose bricks were cut at more than
+// kReplanCutShare of its builds' bricks is planned again from its present
+// cell occupancy.
+constexpr double kDefaultFill = 0.9;
+constexpr double kTuneFills[] = {0.8, 0.9, 1.0};
+constexpr uint32_t kTuneSteps = 6;
+constexpr uint32_t kTuneMinAtoms = 65536;
+constexpr uint64_t kTuneAutoSteps = 64;
+constexpr uint64_t kCheckSteps = 512;
+constexpr double kReplanCutShare = 0.02;
 constexpr double kTwo32 = 4294967296.0;
 constexpr double kPi = 3.14159265358979323846;  // the f64 nearest pi (std::f64::consts::PI)
 
@@ -50,6 +65,7 @@ struct HostReplica {
   double energy_scale, momentum_scale;
   std::vector<uint32_t> brick_x, brick_y;  // brick shape in cells, per z layer
   std::vector<uint8_t> sparse;             // per z layer: its bricks use the sparse layout
+  double headroom = 1.0;  // share of the plan's fill this replica's bricks are held to
   uint64_t prof_samples;      // profile samples since the last clear
 };
 
@@ -81,6 +97,19 @@ struct vlj_sim {
   bool primed = false;
   uint64_t steps = 0;
   long long* fixed_scratch = nullptr;
+  // brick planner: the fill the plans are made with, whether it was tuned,
+  // the trials of the tuning, and the periodic check for replicas whose
+  // bricks are cut (their density rose since they were planned)
+  double fill = kDefaultFill;
+  bool tuned = false;
+  std::vector<vlj::TuneTrial> trials;
+  std::vector<uint32_t> rep_bricks;            // bricks of each replica in the table
+  std::vector<uint32_t> cut_seen, rebuilds_seen;  // counters at the last check
+  Control* check_ctl = nullptr;                // pinned copy of the control words
+  hipEvent_t check_done = nullptr;
+  bool check_in_flight = false;
+  uint64_t last_check = 0;                     // steps at the last check
+  uint32_t replans = 0;                        // re-plans after the first plan
 };
 
 namespace {
@@ -284,6 +313,13 @@ int dev_alloc(vlj_sim* s, T** out, size_t count) {
   return VLJ_OK;
 }
 
+void free_device(vlj_sim* s, void* p) {
+  if (!p) return;
+  auto it = std::find(s->allocations.begin(), s->allocations.end(), p);
+  if (it != s->allocations.end()) s->allocations.erase(it);
+  (void)hipFree(p);
+}
+
 void free_all(vlj_sim* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->graph1) (void)hipGraphExecDestroy(s->graph1);
@@ -293,6 +329,10 @@ void free_all(vlj_sim* s) {
     if (r.ctl) (void)hipHostFree(r.ctl);
     if (r.done) (void)hipEventDestroy(r.done);
   }
+  if (s->check_ctl) (void)hipHostFree(s->check_ctl);
+  s->check_ctl = nullptr;
+  if (s->check_done) (void)hipEventDestroy(s->check_done);
+  s->check_done = nullptr;
   for (void* p : s->allocations) (void)hipFree(p);
   s->allocations.clear();
   if (s->B.hc) (void)hipFree(s->B.hc);
@@ -324,11 +364,15 @@ int check_device(vlj_sim* s) {
 // One step (lennard_jones.rs:1092-1138): with Nose-Hoover replicas a
 // thermostat half step before the drift (its scaling is applied there) and
 // one after the force kick.
-int enqueue_step(vlj_sim* s) {
+// `around_force`, when given, is a pair of events recorded before and after
+// the pair-force launches.
+int enqueue_step(vlj_sim* s, hipEvent_t* around_force = nullptr) {
   if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, false, s->stream));
   HIP_TRY(s, launch_drift(s->B, s->any_langevin, s->stream));
   HIP_TRY(s, launch_rebuild(s->B, s->stream, &s->side));
+  if (around_force) HIP_TRY(s, hipEventRecord(around_force[0], s->stream));
   HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, true, s->stream, &s->side));
+  if (around_force) HIP_TRY(s, hipEventRecord(around_force[1], s->stream));
   if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, true, s->stream));
   if (s->any_heat_exchange) HIP_TRY(s, launch_heat_exchange(s->B, s->stream));
   return VLJ_OK;
@@ -361,41 +405,24 @@ int capture(vlj_sim* s, uint32_t steps, hipGraphExec_t* out) {
   return VLJ_OK;
 }
 
-// lennard_jones.rs:1080-1089: first lists and accelerations.
-int prime_if_needed(vlj_sim* s) {
-  if (s->primed) return VLJ_OK;
-  HIP_TRY(s, launch_decide(s->B, s->stream));
-  HIP_TRY(s, launch_rebuild(s->B, s->stream, &s->side));
-  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, false, s->stream, &s->side));
-  s->primed = true;
-  return VLJ_OK;
-}
-
 // Brick shape of each z layer: the largest gx x gy (home cells, up to
-// kBrickMax each way) whose every brick stages at most 90% of kStageMax
-// atoms. The count a brick is held to is the larger of what it stages in
-// the initial state at its own layer and at the layers above and below (a
-// density profile may drift by a cell, as a liquid-vapour interface does)
-// and the replica's mean occupancy (a lattice that is not yet uniform evens
-// out). Window sums come from a 2D prefix sum of the three
-// layers' counts over the periodically extended plane. A brick that
-// outgrows LDS later on is cut into parts on the device. A layer whose
-// bricks are held to at most 90% of the sparse layout's 1024 atoms takes
-// that layout. vlj_debug_plan may replace the two 90% (a value above 1 plans
-// bricks that will be cut, a negative sparse fill keeps every layer dense:
-// for tests and tuning).
-void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vector<uint32_t>& bx,
-                   std::vector<uint32_t>& by, std::vector<uint8_t>& sparse) {
+// kBrickMax each way) whose every brick stages at most `fill` of kStageMax
+// atoms. The count a brick is held to is the larger of what it stages at
+// its own layer and at the layers above and below (a density profile may
+// drift by a cell, as a liquid-vapour interface does) and the replica's
+// mean occupancy (a lattice that is not yet uniform evens out). The
+// occupancy is the initial state's when a handle is created and that of
+// the last list build when a replica is planned again. Window sums come
+// from a 2D prefix sum of the three layers' counts over the periodically
+// extended plane. A brick that outgrows LDS later on is cut into parts on
+// the device. A layer whose bricks are held to at most `sparse_fill` of the
+// sparse layout's 1024 atoms takes that layout.
+void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fill, double sparse_fill,
+                   std::vector<uint32_t>& bx, std::vector<uint32_t>& by, std::vector<uint8_t>& sparse) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
-#ifndef VLJ_STAGE_FILL
-#define VLJ_STAGE_FILL 0.9
-#endif
 #ifndef VLJ_MEAN_FLOOR
 #define VLJ_MEAN_FLOOR 1.0
 #endif
-  double fill = VLJ_STAGE_FILL, sparse_fill = VLJ_STAGE_FILL;
-  if (g_plan.stage_fill != 0.0) fill = g_plan.stage_fill;
-  if (g_plan.sparse_fill != 0.0) sparse_fill = g_plan.sparse_fill;
   const double limit = fill * kStageMax;
   const double sparse_limit = sparse_fill * Sparse::kStageMax;
   const double mean = VLJ_MEAN_FLOOR * (double)R.count / (double)R.total_cells;
@@ -424,6 +451,10 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
     return P[(size_t)yb * (ex + 1) + xb] - P[(size_t)ya * (ex + 1) + xb] - P[(size_t)yb * (ex + 1) + xa] +
            P[(size_t)ya * (ex + 1) + xa];
   };
+  // A count held to the limit is the count plus four of its Poisson
+  // standard deviations: a liquid's windows fluctuate about what the plan
+  // saw (a lattice at the start), and a brick that outgrows LDS is cut.
+  auto margin = [](uint64_t n) { return (double)n + 4.0 * std::sqrt((double)n); };
   bx.assign(nz, 1);
   by.assign(nz, 1);
   sparse.assign(nz, 0);
@@ -434,12 +465,12 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
       for (int gx = gy; gx <= std::min(kBrickMax, nx); gx++) {
         const uint32_t home = (uint32_t)(gx * gy);
         if (home <= best) continue;
-        double worst = mean * 3.0 * (gx + 2) * (gy + 2);
+        double worst = margin((uint64_t)(mean * 3.0 * (gx + 2) * (gy + 2)));
         for (int dz = -1; dz <= 1 && worst <= limit; dz++) {
           const int zz = (z + dz + nz) % nz;
           for (int y0 = 0; y0 < ny && worst <= limit; y0 += gy)
             for (int x0 = 0; x0 < nx && worst <= limit; x0 += gx)
-              worst = std::max(worst, (double)window(zz, x0, y0, std::min(gx, nx - x0), std::min(gy, ny - y0)));
+              worst = std::max(worst, margin(window(zz, x0, y0, std::min(gx, nx - x0), std::min(gy, ny - y0))));
         }
         if (worst <= limit) {
           best = home;
@@ -451,6 +482,271 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, std::vect
     // a layer whose bricks stage few atoms takes the sparse layout
     sparse[z] = best > 0 && best_worst <= sparse_limit ? 1 : 0;
   }
+}
+
+// The fills a plan is made with: the handle's (tuned) fill unless
+// vlj_debug_plan overrides it; the sparse layout's threshold follows the
+// dense one unless overridden (a negative override keeps every layer dense).
+double plan_fill(const vlj_sim* s) { return g_plan.stage_fill != 0.0 ? g_plan.stage_fill : s->fill; }
+double plan_sparse_fill(const vlj_sim* s) {
+  return g_plan.sparse_fill != 0.0 ? g_plan.sparse_fill : plan_fill(s);
+}
+
+// The brick table from the replicas' brick shapes: every replica's dense
+// bricks, then every sparse one, each part with its larger bricks (more
+// home cells) first and otherwise in layer, row and column order, so that
+// the last workgroups of a launch, which run while the rest of the GPU has
+// nothing left to start, are the smallest, and neighbouring bricks still
+// run side by side (sharing their halos in L2). The 1M-atom force kernel:
+// 154 -> 146 us; the 512-replica batch +3.7%. Ordering by each brick's
+// estimated pair work instead scattered neighbours: 149 us, +2.6%.
+void build_table(vlj_sim* s) {
+  s->bricks.clear();
+  s->rep_bricks.assign(s->rep.size(), 0u);
+  for (int pass = 0; pass < 2; pass++) {
+    const size_t pass_start = s->bricks.size();
+    for (size_t r = 0; r < s->rep.size(); r++) {
+      const Replica& R = s->rep[r];
+      const HostReplica& H = s->host[r];
+      for (uint32_t z = 0; z < R.cells[2]; z++) {
+        if ((H.sparse[z] != 0) != (pass == 1)) continue;
+        const uint32_t gx = H.brick_x[z], gy = H.brick_y[z];
+        for (uint32_t y0 = 0; y0 < R.cells[1]; y0 += gy)
+          for (uint32_t x0 = 0; x0 < R.cells[0]; x0 += gx) {
+            s->bricks.push_back({(uint32_t)r, (z * R.cells[1] + y0) * R.cells[0] + x0,
+                                 std::min(gx, R.cells[0] - x0), std::min(gy, R.cells[1] - y0)});
+            s->rep_bricks[r]++;
+          }
+      }
+    }
+    std::stable_sort(s->bricks.begin() + (std::ptrdiff_t)pass_start, s->bricks.end(),
+                     [](const Brick& a, const Brick& b) { return a.gx * a.gy > b.gx * b.gy; });
+    if (pass == 0) s->n_dense = (uint32_t)s->bricks.size();
+  }
+}
+
+// lennard_jones.rs:1080-1089: first lists and accelerations.
+int prime_if_needed(vlj_sim* s) {
+  if (s->primed) return VLJ_OK;
+  HIP_TRY(s, launch_decide(s->B, s->stream));
+  HIP_TRY(s, launch_rebuild(s->B, s->stream, &s->side));
+  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, false, s->stream, &s->side));
+  s->primed = true;
+  return VLJ_OK;
+}
+
+// The side stream for the sparse-layout launches, once both layouts have
+// bricks (default priority).
+int ensure_side(vlj_sim* s) {
+  if (s->side.stream || s->n_dense == 0 || s->n_dense == s->bricks.size()) return VLJ_OK;
+  HIP_TRY(s, hipStreamCreateWithFlags(&s->side.stream, hipStreamNonBlocking));
+  HIP_TRY(s, hipEventCreateWithFlags(&s->side.fork, hipEventDisableTiming));
+  HIP_TRY(s, hipEventCreateWithFlags(&s->side.join, hipEventDisableTiming));
+  return VLJ_OK;
+}
+
+// Puts the brick table s->bricks on the device (new buffers when its size
+// changed) with every cut mark cleared; the captured steps hold the old
+// table's size and pointers, so they are captured again.
+int upload_table(vlj_sim* s) {
+  Buffers& B = s->B;
+  const size_t nb = s->bricks.size(), ng = (nb + kCutGroup - 1) / kCutGroup;
+  drop_graphs(s);
+  if (nb != B.n_bricks) {
+    free_device(s, (void*)B.bricks);
+    free_device(s, B.cut);
+    free_device(s, B.parts);
+    free_device(s, B.cut_groups);
+    B.bricks = nullptr;
+    B.cut = nullptr;
+    B.parts = nullptr;
+    B.cut_groups = nullptr;
+    Brick* gb = nullptr;
+    int rc = dev_alloc(s, &gb, nb);
+    if (rc == VLJ_OK) rc = dev_alloc(s, &B.cut, nb);
+    if (rc == VLJ_OK) rc = dev_alloc(s, &B.parts, nb * kHomeCells);
+    if (rc == VLJ_OK) rc = dev_alloc(s, &B.cut_groups, ng);
+    if (rc != VLJ_OK) return rc;
+    B.bricks = gb;
+  }
+  B.n_bricks = (uint32_t)nb;
+  B.n_dense = s->n_dense;
+  HIP_TRY(s, hipMemcpy((void*)B.bricks, s->bricks.data(), nb * sizeof(Brick), hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemset(B.cut, 0, std::max<size_t>(nb, 1) * 4));
+  HIP_TRY(s, hipMemset(B.cut_groups, 0, std::max<size_t>(ng, 1) * 4));
+  HIP_TRY(s, hipMemset(B.flags + 1, 0, 4));
+  return ensure_side(s);
+}
+
+// Plans the bricks of the replicas marked in `which` again with `fill`,
+// from the cell occupancy of their last list build, and builds every list
+// again from the positions of that build (launch_relist): the same lists,
+// in the same order, staged for the new bricks, so the trajectory does not
+// change. *build_ms, when given, gets the time of that list build.
+int replan(vlj_sim* s, const std::vector<char>& which, double fill, double* build_ms = nullptr) {
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  std::vector<uint32_t> cn(std::max<size_t>(s->cells_total, 1));
+  if (s->cells_total) HIP_TRY(s, hipMemcpy(cn.data(), s->B.cell_n, s->cells_total * 4, hipMemcpyDeviceToHost));
+  const double saved = s->fill;
+  s->fill = fill;
+  for (size_t r = 0; r < s->rep.size(); r++) {
+    if (!which[r]) continue;
+    const Replica& R = s->rep[r];
+    std::vector<uint32_t> occ(cn.begin() + R.cell_off, cn.begin() + R.cell_off + R.total_cells);
+    HostReplica& H = s->host[r];
+    choose_bricks(R, occ, plan_fill(s) * H.headroom, plan_sparse_fill(s) * H.headroom, H.brick_x, H.brick_y,
+                  H.sparse);
+  }
+  if (g_plan.stage_fill != 0.0) s->fill = saved;
+  build_table(s);
+  int rc = upload_table(s);
+  if (rc != VLJ_OK) return rc;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (build_ms) {
+    HIP_TRY(s, hipEventCreate(&ev[0]));
+    HIP_TRY(s, hipEventCreate(&ev[1]));
+    HIP_TRY(s, hipEventRecord(ev[0], s->stream));
+  }
+  HIP_TRY(s, launch_relist(s->B, s->stream));
+  if (build_ms) HIP_TRY(s, hipEventRecord(ev[1], s->stream));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  if (build_ms) {
+    float ms = 0.0f;
+    HIP_TRY(s, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    *build_ms = ms;
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+  }
+  return VLJ_OK;
+}
+
+// The planner's tuner: for each candidate fill, plan every replica's bricks
+// with it (replan), then run `per` steps launching the kernels directly
+// with HIP events around the pair-force launches; the score is the mean
+// force time per step plus the list build's time times the rebuilds per
+// step seen over the trials. The fastest plan is kept. The steps are real
+// steps of the run, and bricks do not change results, so tuning changes no
+// bit of the trajectory. Returns the steps taken.
+int tune(vlj_sim* s, uint32_t per, uint64_t* taken) {
+  *taken = 0;
+  int rc = prime_if_needed(s);
+  if (rc != VLJ_OK) return rc;
+  const std::vector<char> all(s->rep.size(), 1);
+  std::vector<hipEvent_t> ev(2 * per, nullptr);
+  for (auto& e : ev) HIP_TRY(s, hipEventCreate(&e));
+  auto rebuilds = [&](uint64_t* out) {
+    std::vector<Control> c(s->rep.size());
+    HIP_TRY(s, hipMemcpy(c.data(), s->B.ctl, c.size() * sizeof(Control), hipMemcpyDeviceToHost));
+    *out = 0;
+    for (const Control& x : c) *out += x.rebuilds;
+    return VLJ_OK;
+  };
+  uint64_t r0 = 0, r1 = 0;
+  rc = rebuilds(&r0);
+  s->trials.clear();
+  for (double f : kTuneFills) {
+    double build_ms = 0.0;
+    if (rc == VLJ_OK) rc = replan(s, all, f, &build_ms);
+    double force_ms = 0.0;
+    for (uint32_t t = 0; t < per && rc == VLJ_OK; t++) rc = enqueue_step(s, &ev[2 * t]);
+    if (rc != VLJ_OK) break;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    for (uint32_t t = 0; t < per; t++) {
+      float ms = 0.0f;
+      HIP_TRY(s, hipEventElapsedTime(&ms, ev[2 * t], ev[2 * t + 1]));
+      force_ms += ms;
+    }
+    s->steps += per;
+    *taken += per;
+    s->trials.push_back({f, 1e3 * force_ms / per, 1e3 * build_ms, (uint32_t)s->bricks.size()});
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc == VLJ_OK) rc = rebuilds(&r1);
+  if (rc != VLJ_OK) return rc;
+  const double per_step = std::max(1.0 / 16.0, (double)(r1 - r0) / std::max<double>(1.0, (double)*taken)) /
+                          (double)std::max<size_t>(s->rep.size(), 1);
+  size_t best = 0;
+  for (size_t k = 1; k < s->trials.size(); k++) {
+    const TuneTrial& a = s->trials[k];
+    const TuneTrial& b = s->trials[best];
+    if (a.force_us + per_step * a.build_us < b.force_us + per_step * b.build_us) best = k;
+  }
+  s->tuned = true;
+  if (s->trials.empty()) return VLJ_OK;
+  if (s->trials[best].fill != s->trials.back().fill) return replan(s, all, s->trials[best].fill);
+  s->fill = s->trials.back().fill;
+  return VLJ_OK;
+}
+
+bool tune_wanted(const vlj_sim* s) {
+  return !s->tuned && g_plan.stage_fill == 0.0 && s->n_total >= kTuneMinAtoms && !s->bricks.empty();
+}
+
+// The periodic check: takes the control words copied back at the last
+// check, if they have arrived, and plans again every replica whose bricks
+// were cut at more than kReplanCutShare of its bricks per list build since
+// the check before (when that changes its plan); then starts the next copy
+// once kCheckSteps steps have passed. Never waits for the device except
+// to plan again.
+int check_bricks(vlj_sim* s) {
+  if (g_plan.stage_fill != 0.0 || s->bricks.empty()) return VLJ_OK;
+  const size_t R = s->rep.size();
+  if (!s->check_ctl) {
+    HIP_TRY(s, hipHostMalloc((void**)&s->check_ctl, R * sizeof(Control), hipHostMallocDefault));
+    HIP_TRY(s, hipEventCreateWithFlags(&s->check_done, hipEventDisableTiming));
+    s->cut_seen.assign(R, 0u);
+    s->rebuilds_seen.assign(R, 0u);
+  }
+  if (s->check_in_flight) {
+    hipError_t e = hipEventQuery(s->check_done);
+    if (e == hipErrorNotReady) return VLJ_OK;
+    if (e != hipSuccess) return fail(s, VLJ_E_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+    s->check_in_flight = false;
+    std::vector<char> which(R, 0);
+    bool any = false;
+    for (size_t r = 0; r < R; r++) {
+      const Control& c = s->check_ctl[r];
+      const uint32_t dcut = c.stage_overflow - s->cut_seen[r], dreb = c.rebuilds - s->rebuilds_seen[r];
+      s->cut_seen[r] = c.stage_overflow;
+      s->rebuilds_seen[r] = c.rebuilds;
+      if (dreb == 0 || (double)dcut <= kReplanCutShare * s->rep_bricks[r] * dreb) continue;
+      // plan it from the present occupancy only if that gives other bricks
+      which[r] = 1;
+      any = true;
+    }
+    if (any) {
+      std::vector<uint32_t> cn(s->cells_total);
+      HIP_TRY(s, hipStreamSynchronize(s->stream));
+      HIP_TRY(s, hipMemcpy(cn.data(), s->B.cell_n, s->cells_total * 4, hipMemcpyDeviceToHost));
+      any = false;
+      for (size_t r = 0; r < R; r++) {
+        if (!which[r]) continue;
+        const Replica& rp = s->rep[r];
+        std::vector<uint32_t> occ(cn.begin() + rp.cell_off, cn.begin() + rp.cell_off + rp.total_cells);
+        // a replica whose bricks keep being cut is held to 10% less of the
+        // staging limit each time (down to 60%)
+        HostReplica& H = s->host[r];
+        H.headroom = std::max(0.6, 0.9 * H.headroom);
+        std::vector<uint32_t> bx, by;
+        std::vector<uint8_t> sp;
+        choose_bricks(rp, occ, plan_fill(s) * H.headroom, plan_sparse_fill(s) * H.headroom, bx, by, sp);
+        which[r] = bx != H.brick_x || by != H.brick_y || sp != H.sparse;
+        any = any || which[r];
+      }
+      if (any) {
+        int rc = replan(s, which, s->fill);
+        if (rc != VLJ_OK) return rc;
+        s->replans++;
+      }
+    }
+  }
+  if (s->steps >= s->last_check + kCheckSteps) {
+    HIP_TRY(s, hipMemcpyAsync(s->check_ctl, s->B.ctl, R * sizeof(Control), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipEventRecord(s->check_done, s->stream));
+    s->check_in_flight = true;
+    s->last_check = s->steps;
+  }
+  return VLJ_OK;
 }
 
 // Builds a handle over `replicas` systems.
@@ -515,7 +811,8 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     }
     R.atom_block_off = (uint32_t)s->atom_blocks.size();
     R.atom_blocks = (R.count + kBlock - 1) / kBlock;
-    choose_bricks(R, occ, s->host[r].brick_x, s->host[r].brick_y, s->host[r].sparse);
+    choose_bricks(R, occ, plan_fill(s), plan_sparse_fill(s), s->host[r].brick_x, s->host[r].brick_y,
+                  s->host[r].sparse);
     for (uint32_t b = 0; b < R.count; b += kBlock)
       s->atom_blocks.push_back({(uint32_t)r, atom_off + b, atom_off + std::min(R.count, b + kBlock), 0});
     for (uint32_t b = 0; b < R.total_cells; b += R.scan_cells)
@@ -532,32 +829,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     if (R.thermostat == 1) s->any_langevin = true;
     if (R.thermostat == 2) s->any_nose_hoover = true;
   }
-  // The brick table: every replica's dense bricks, then every sparse one,
-  // each part with its larger bricks (more home cells) first and otherwise
-  // in layer, row and column order, so that the last workgroups of a
-  // launch, which run while the rest of the GPU has nothing left to start,
-  // are the smallest, and neighbouring bricks still run side by side
-  // (sharing their halos in L2). The 1M-atom force kernel: 154 -> 146 us;
-  // the 512-replica batch +3.7%. Ordering by each brick's estimated pair
-  // work instead scattered neighbours: 149 us, +2.6%.
-  for (int pass = 0; pass < 2; pass++) {
-    const size_t pass_start = s->bricks.size();
-    for (size_t r = 0; r < replicas; r++) {
-      const Replica& R = s->rep[r];
-      const HostReplica& H = s->host[r];
-      for (uint32_t z = 0; z < R.cells[2]; z++) {
-        if ((H.sparse[z] != 0) != (pass == 1)) continue;
-        const uint32_t gx = H.brick_x[z], gy = H.brick_y[z];
-        for (uint32_t y0 = 0; y0 < R.cells[1]; y0 += gy)
-          for (uint32_t x0 = 0; x0 < R.cells[0]; x0 += gx)
-            s->bricks.push_back({(uint32_t)r, (z * R.cells[1] + y0) * R.cells[0] + x0,
-                                 std::min(gx, R.cells[0] - x0), std::min(gy, R.cells[1] - y0)});
-      }
-    }
-    std::stable_sort(s->bricks.begin() + (std::ptrdiff_t)pass_start, s->bricks.end(),
-                     [](const Brick& a, const Brick& b) { return a.gx * a.gy > b.gx * b.gy; });
-    if (pass == 0) s->n_dense = (uint32_t)s->bricks.size();
-  }
+  build_table(s);
   s->n_total = atom_off;
   s->cells_total = cell_off;
   s->list_total = list_off;
@@ -712,6 +984,14 @@ int enqueue_steps(vlj_sim* s, uint64_t steps) {
     s->steps += steps;
     return VLJ_OK;
   }
+  if (tune_wanted(s) && steps >= kTuneAutoSteps) {
+    uint64_t taken = 0;
+    rc = tune(s, kTuneSteps, &taken);
+    if (rc != VLJ_OK) return rc;
+    steps -= taken;
+  }
+  rc = check_bricks(s);
+  if (rc != VLJ_OK) return rc;
   rc = ensure_graphs(s);
   if (rc != VLJ_OK) return rc;
   uint64_t left = steps;
@@ -724,7 +1004,7 @@ int enqueue_steps(vlj_sim* s, uint64_t steps) {
     left--;
   }
   s->steps += steps;
-  return VLJ_OK;
+  return check_bricks(s);
 }
 
 void decode(const vlj_sim* s, uint32_t r, const long long* slots, const Control& c, uint64_t step,
@@ -1363,6 +1643,29 @@ int vlj_debug_params(const vlj_sim* sim, double out[6]) {
   return VLJ_OK;
 }
 
+int vlj_tune(vlj_sim* sim, uint32_t steps_per_candidate, uint64_t* steps_taken) {
+  if (!sim || steps_per_candidate == 0) return fail(sim, VLJ_E_INVALID, "bad arguments to vlj_tune");
+  if (g_plan.stage_fill != 0.0) return fail(sim, VLJ_E_STATE, "the brick plan is fixed by vlj_debug_plan");
+  uint64_t taken = 0;
+  int rc = VLJ_OK;
+  if (sim->n_total && !sim->bricks.empty()) rc = tune(sim, steps_per_candidate, &taken);
+  if (steps_taken) *steps_taken = taken;
+  return rc;
+}
+
+int vlj_plan_info(const vlj_sim* sim, double* fill, uint32_t* replans, vlj_tune_trial* trials, size_t capacity,
+                  size_t* n_trials) {
+  if (!sim || (capacity && !trials)) return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_plan_info");
+  if (fill) *fill = plan_fill(sim);
+  if (replans) *replans = sim->replans;
+  for (size_t k = 0; k < std::min(capacity, sim->trials.size()); k++) {
+    const TuneTrial& t = sim->trials[k];
+    trials[k] = vlj_tune_trial{t.fill, t.force_us, t.build_us, t.bricks};
+  }
+  if (n_trials) *n_trials = sim->trials.size();
+  return VLJ_OK;
+}
+
 int vlj_debug_plan(uint32_t list_capacity, double stage_fill, double sparse_fill) {
   if (!std::isfinite(stage_fill) || !std::isfinite(sparse_fill) || stage_fill < 0.0)
     return fail(nullptr, VLJ_E_INVALID, "bad arguments to vlj_debug_plan");
@@ -1784,13 +2087,6 @@ double* const* state_f64(Buffers& B) {
   return p;
 }
 
-void free_device(vlj_sim* s, void* p) {
-  if (!p) return;
-  auto it = std::find(s->allocations.begin(), s->allocations.end(), p);
-  if (it != s->allocations.end()) s->allocations.erase(it);
-  (void)hipFree(p);
-}
-
 }  // namespace
 
 extern "C" {
@@ -2020,6 +2316,9 @@ int vlj_load(const char* path, vlj_sim** out) {
   s->bricks = bricks;
   s->n_dense = h.n_dense;
   B.n_dense = h.n_dense;
+  s->rep_bricks.assign(R, 0u);
+  for (const Brick& g : bricks) s->rep_bricks[g.rep]++;
+  s->tuned = true;  // a loaded handle keeps the saved plan
   s->rep = rep;
   for (size_t k = 0; k < R; k++) {
     s->host[k].prof_samples = prof_samples[k];

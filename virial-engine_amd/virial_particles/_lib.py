@@ -66,6 +66,10 @@ class Measurement(C.Structure):
     ]
 
 
+class TuneTrial(C.Structure):
+    _fields_ = [("fill", C.c_double), ("force_us", C.c_double), ("build_us", C.c_double), ("bricks", C.c_uint32)]
+
+
 # Every symbol include/virial_lj.h declares, with its ctypes signature.
 _P = C.c_void_p
 _d = C.c_double
@@ -121,6 +125,8 @@ SIGNATURES = {
     "vlj_batch_measure_now": (C.c_int, [_P, C.POINTER(Measurement), _sz]),
     "vlj_batch_replicas": (_sz, [_P]),
     "vlj_batch_read_particles": (C.c_int, [_P, _pp, _sz]),
+    "vlj_tune": (C.c_int, [_P, _u32, _u64p]),
+    "vlj_plan_info": (C.c_int, [_P, _dp, _up, C.POINTER(TuneTrial), _sz, C.POINTER(_sz)]),
     "vlj_lattice": (C.c_int, [_sz, _d, _d, _u64, _pp]),
     "vlj_lattice_box": (C.c_int, [_sz, _dp, _d, _u64, _pp]),
     "vlj_fcc": (C.c_int, [_u32, _d, _d, _u64, _pp, _dp]),

@@ -608,11 +608,38 @@ class LennardJones:
         self._c(L.lib().vlj_debug_lists(self._h, _uptr(counts), _uptr(entries)))
         return counts, entries
 
+    def tune(self, steps_per_candidate: int = 6) -> int:
+        """Times each candidate brick plan over real steps and keeps the
+        fastest (vlj_tune); returns the steps it ran."""
+        return _tune(self._h, steps_per_candidate)
+
+    def plan_info(self) -> dict:
+        """The fill of the brick plan in use, re-plans so far and the last
+        tuning's trials."""
+        return _plan_info(self._h)
+
     def timed_run(self, steps: int):
         """(force-kernel ms summed over the steps, total ms) from HIP events."""
         f, t = C.c_double(), C.c_double()
         self._c(L.lib().vlj_timed_run(self._h, steps, C.byref(f), C.byref(t)))
         return f.value, t.value
+
+
+def _tune(h, steps_per_candidate: int) -> int:
+    taken = C.c_uint64(0)
+    _check(L.lib().vlj_tune(h, steps_per_candidate, C.byref(taken)), h)
+    return taken.value
+
+
+def _plan_info(h) -> dict:
+    fill = C.c_double()
+    replans = C.c_uint32()
+    n = C.c_size_t()
+    trials = (L.TuneTrial * 16)()
+    _check(L.lib().vlj_plan_info(h, C.byref(fill), C.byref(replans), trials, 16, C.byref(n)), h)
+    return {"fill": fill.value, "replans": replans.value,
+            "trials": [{"fill": t.fill, "force_us": t.force_us, "build_us": t.build_us, "bricks": t.bricks}
+                       for t in trials[: min(n.value, 16)]]}
 
 
 def _debug_counters(h) -> dict:
@@ -670,6 +697,14 @@ class Batch:
 
     def run(self, steps: int) -> None:
         _check(L.lib().vlj_batch_run(self._h, steps), self._h)
+
+    def tune(self, steps_per_candidate: int = 6) -> int:
+        """Times each candidate brick plan over real steps and keeps the
+        fastest (vlj_tune); returns the steps it ran."""
+        return _tune(self._h, steps_per_candidate)
+
+    def plan_info(self) -> dict:
+        return _plan_info(self._h)
 
     def set_temperature(self, replica: int, temperature: float) -> None:
         """set_temperature (lennard_jones.rs:905-925) of one replica."""
