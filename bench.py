@@ -399,6 +399,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--batch-table", help="write each replica's PV/NkT and standard error here (JSON)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched form")
+    ap.add_argument("--physics", action="store_true",
+                    help="the reference's physical validations, full protocols (JSON per check)")
     ap.add_argument("--summarize", metavar="ROCPROF_DIR")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--out")
@@ -418,6 +420,20 @@ def main():
     from virial_particles import lennard_jones as lj
     from virial_particles import workloads as W
 
+    if args.physics:
+        from virial_particles import validation as V
+
+        checks = [("mb_relaxation", V.mb_relaxation, {}),
+                  ("dt_study", V.dt_study, {}),
+                  ("coexistence_force_shifted", V.coexistence, {"shifted_force": True, "blocks": 5, "equilibrate": 60_000}),
+                  ("coexistence_truncated", V.coexistence, {"shifted_force": False, "blocks": 5, "equilibrate": 60_000}),
+                  ("fourier_green_kubo", V.fourier_green_kubo, {})]
+        for name, f, kw in checks:
+            t0 = time.perf_counter()
+            r = f(**kw)
+            r["seconds"] = time.perf_counter() - t0
+            print(json.dumps({"check": name, **r}, default=str), flush=True)
+        return
     if args.slab:
         print(json.dumps(run_slab(lj, args.slab_cells, 8, args.steps, args.warmup, 100, 200)), flush=True)
         return

@@ -698,6 +698,14 @@ class Batch:
     def run(self, steps: int) -> None:
         _check(L.lib().vlj_batch_run(self._h, steps), self._h)
 
+    def step(self, steps: int) -> None:
+        """Enqueues `steps` steps of every replica and returns at once."""
+        _check(L.lib().vlj_step(self._h, steps), self._h)
+
+    def clear_profile(self) -> None:
+        """Starts every replica's profile sums afresh."""
+        _check(L.lib().vlj_clear_profile(self._h), self._h)
+
     def tune(self, steps_per_candidate: int = 6) -> int:
         """Times each candidate brick plan over real steps and keeps the
         fastest (vlj_tune); returns the steps it ran."""
