@@ -1351,6 +1351,26 @@ __device__ __forceinline__ u32x4 load_octet(const uint16_t* base, uint32_t o, ui
   return __builtin_nontemporal_load((const u32x4*)(base + (uint64_t)min(o, last) * (8u * kListStride)));
 }
 
+// f(k, entry) for entries k = 0 .. cnt - 1 of a list in order, read an
+// octet (16 bytes) at a time with the next octet's load in flight, so that
+// a loop over the entries does not wait on one 2-byte load per entry.
+template <class F>
+__device__ __forceinline__ void each_entry(const uint16_t* lst, uint32_t cnt, F&& f) {
+  const uint32_t no = (cnt + 7u) / 8u;
+  const uint32_t last = no > 0u ? no - 1u : 0u;
+  u32x4 cur = load_octet(lst, 0u, last);
+  for (uint32_t o = 0; o < no; o++) {
+    const u32x4 nxt = load_octet(lst, o + 1u, last);
+    const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
+#pragma unroll
+    for (uint32_t c = 0; c < 8u; c++) {
+      const uint32_t k = 8u * o + c;
+      if (k < cnt) f(k, (w[c >> 1] >> ((c & 1u) * 16u)) & 0xffffu);
+    }
+    cur = nxt;
+  }
+}
+
 // Atom i's list in order, kOctetBatch octets at a time, double-buffered:
 // the next batch's loads are in flight while this one is summed. Positions
 // come from the brick's LDS; whole octets are taken (pads meet the far
@@ -1417,9 +1437,9 @@ __device__ __forceinline__ PairAcc interactions_global(const Buffers& B, const R
   const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
   PairAcc a = {0.0, 0.0, 0.0, 0.0, 0.0, 0u, 0u};
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;  // FP32 forces: the current octet's sum
-  for (uint32_t k = 0; k < cnt; k++) {
+  each_entry(lst, cnt, [&](uint32_t k, uint32_t entry) {
     double xj, yj, zj;
-    staged_pos(R, S, false, list_index(lst[list_entry(k)], false), B.x, B.y, B.z, xj, yj, zj);
+    staged_pos(R, S, false, list_index(entry, false), B.x, B.y, B.z, xj, yj, zj);
     const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
     if constexpr (PREC == 1 && !ENERGIES) {
       pair_force_f32<SF>(R, dx, dy, dz, dist2(dx, dy, dz), fx, fy, fz);
@@ -1432,7 +1452,7 @@ __device__ __forceinline__ PairAcc interactions_global(const Buffers& B, const R
     } else {
       pair_accumulate<PREC, SF, ENERGIES>(R, dx, dy, dz, dist2(dx, dy, dz), a);
     }
-  }
+  });
   return a;
 }
 
@@ -1606,6 +1626,34 @@ struct ProfileSink {
   __device__ __forceinline__ void add(uint32_t k, double x) const { add(k, to_fixed(x, kProfileOne)); }
 };
 
+// The virial shares of one atom's pairs to the slab below its own slab,
+// its own and the one above, each share already in fixed point: added up
+// in registers (integer sums, so the order does not matter) and sent to the
+// profile once per atom instead of once per share. With slabs at least a
+// cutoff wide every share lands in these three; a share elsewhere (narrow
+// slabs) goes to the profile at once.
+struct SlabWindow {
+  long long w[3][3];  // [slab below, own, above][axis]
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) w[k][a] = 0;
+  }
+  // k: the share's slab relative to the atom's (-1, 0, 1 fit)
+  __device__ __forceinline__ void add(int k, const long long f[3], const ProfileSink& sink, uint32_t slab) {
+    if (k >= -1 && k <= 1) {
+#pragma unroll
+      for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) w[kk][a] += kk == k + 1 ? f[a] : 0ll;
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; a++) sink.add(slab * kProfileSums + 4u + (uint32_t)a, f[a]);
+    }
+  }
+};
+
 template <bool SF, class L>
 __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                              bool staged, const ProfileSink& sink, long long whole[3]) {
@@ -1619,21 +1667,23 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
     double xi, yi, zi;
     staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
     const double zw = wrap_one(zi, R.box[2]);
-    const int hb = (int)floor(zw / width);
+    const int hb = (int)floor(zw / width);  // the slab the shares are counted from
     const uint32_t home = (uint32_t)(hb < 0 ? 0 : (hb > ibins - 1 ? ibins - 1 : hb));
     sink.add(home * kProfileSums, 1.0);
     const double v[3] = {B.vx[i], B.vy[i], B.vz[i]};
 #pragma unroll
     for (int a = 0; a < 3; a++) sink.add(home * kProfileSums + 1u + (uint32_t)a, v[a] * v[a]);
     double pe = 0.0, total[3] = {0.0, 0.0, 0.0};
+    SlabWindow win;
+    win.clear();
     const uint32_t cnt = B.nbr_count[i];
     const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
-    for (uint32_t k = 0; k < cnt; k++) {
+    each_entry(lst, cnt, [&](uint32_t k, uint32_t entry) {
       double xj, yj, zj;
-      staged_pos(R, S, staged, list_index(lst[list_entry(k)], staged), B.x, B.y, B.z, xj, yj, zj);
+      staged_pos(R, S, staged, list_index(entry, staged), B.x, B.y, B.z, xj, yj, zj);
       const double d[3] = {xj - xi, yj - yi, zj - zi};
       const double r2 = dist2(d[0], d[1], d[2]);
-      if (!(r2 < R.rc2)) continue;
+      if (!(r2 < R.rc2)) return;
       const double inv2 = 1.0 / r2;
       const double s6 = inv2 * inv2 * inv2;
       double rf = s6 * __builtin_fma(48.0, s6, -24.0);
@@ -1655,10 +1705,12 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
       const double lo = zw + (d[2] < 0.0 ? d[2] : 0.0);
       const double hi = zw + (d[2] > 0.0 ? d[2] : 0.0);
       const double length = hi - lo;
+      long long f[3];
       if (length < 1e-6 * width) {
 #pragma unroll
-        for (int a = 0; a < 3; a++) sink.add(home * kProfileSums + 4u + (uint32_t)a, w[a]);
-        continue;
+        for (int a = 0; a < 3; a++) f[a] = to_fixed(w[a], kProfileOne);
+        win.add((int)home - hb, f, sink, home);
+        return;
       }
       int b = (int)floor(lo / width);
       double left = 1.0;
@@ -1672,14 +1724,181 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
         left -= share;
         const uint32_t slab = (uint32_t)(((b % ibins) + ibins) % ibins);
 #pragma unroll
-        for (int a = 0; a < 3; a++) sink.add(slab * kProfileSums + 4u + (uint32_t)a, w[a] * share);
+        for (int a = 0; a < 3; a++) f[a] = to_fixed(w[a] * share, kProfileOne);
+        win.add(b - hb, f, sink, slab);
         if (last) break;
         b++;
       }
+    });
+    // the window's sums, to the slabs below, at and above the atom's
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++) {
+      const uint32_t slab = (uint32_t)((((hb - 1 + kk) % ibins) + ibins) % ibins);
+#pragma unroll
+      for (int a = 0; a < 3; a++) sink.add(slab * kProfileSums + 4u + (uint32_t)a, win.w[kk][a]);
     }
     sink.add(home * kProfileSums + 7u, pe);
 #pragma unroll
     for (int a = 0; a < 3; a++) whole[a] += to_fixed(total[a], kProfileOne);
+  }
+}
+
+// Slabs at least a cutoff wide: a wave's atoms put all their items in the
+// slabs hmin - 1 .. hmin + kWaveSlots - 2, hmin the lowest home slab in the
+// wave (a brick covers one cell layer, so a wave's home slabs span one or
+// two). Each (slab, sum) of these is added over the wave's lanes first and
+// sent to the profile by one atomic, with no LDS bins (two workgroups per
+// CU instead of one). A wave whose home slabs spread wider sends each
+// lane's items on their own.
+constexpr int kWaveSlots = 5;
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+template <bool SF, class L>
+__device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replica& R, const Stage<L>& S,
+                                                  const Brick& g, bool staged, long long* prof, long long whole[3]) {
+  const uint32_t bins = R.prof_bins;
+  const double width = R.bin_width;
+  const int ibins = (int)bins;
+  const ProfileSink direct{nullptr, prof, false};
+  const uint32_t H = S.hoff[g.gx * g.gy];
+  const uint32_t lane = threadIdx.x & 63u;
+  // one wave-wide sum per slot of sum q, the lanes' values from `value`
+  auto send = [&](int hmin, int q, auto value) {
+#pragma unroll
+    for (int rs = 0; rs < kWaveSlots; rs++) {
+      const long long x = wave_sum_i64(value(rs));
+      if (lane == 0 && x != 0) {
+        const uint32_t slab = (uint32_t)((((hmin - 1 + rs) % ibins) + ibins) % ibins);
+        atomicAdd((unsigned long long*)(prof + slab * kProfileSums + (uint32_t)q), (unsigned long long)x);
+      }
+    }
+  };
+  for (uint32_t h0 = 0; h0 < H; h0 += L::kThreads) {
+    const uint32_t h = h0 + threadIdx.x;
+    const bool active = h < H;
+    Home at{0, 0, 0, 0};
+    double xi = 0.0, yi = 0.0, zi = 0.0, zw = 0.0;
+    int hb = 0x3fffffff;
+    uint32_t home = 0;
+    if (active) {
+      at = home_atom(g, S, h);
+      staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
+      zw = wrap_one(zi, R.box[2]);
+      hb = (int)floor(zw / width);
+      home = (uint32_t)(hb < 0 ? 0 : (hb > ibins - 1 ? ibins - 1 : hb));
+    }
+    const int hmin = wave_min_i32(hb);
+    if (hmin == 0x3fffffff) continue;  // no atom in this wave
+    const int hmax = -wave_min_i32(active ? -hb : -hmin);
+    const bool together = hmax - hmin <= kWaveSlots - 3;  // the same for the whole wave
+    const int hslot = (int)home - hmin + 1;  // this lane's home slab among the slots
+    const int wslot = hb - hmin;             // ... and its window's first slab
+    // count and v_a^2 to the home slab
+    long long item[4] = {0, 0, 0, 0};
+    if (active) {
+      const uint32_t i = at.slot;
+      const double v[3] = {B.vx[i], B.vy[i], B.vz[i]};
+      item[0] = to_fixed(1.0, kProfileOne);
+#pragma unroll
+      for (int a = 0; a < 3; a++) item[1 + a] = to_fixed(v[a] * v[a], kProfileOne);
+    }
+    if (together) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) send(hmin, q, [&](int rs) { return active && hslot == rs ? item[q] : 0ll; });
+    } else if (active) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) direct.add(home * kProfileSums + (uint32_t)q, item[q]);
+    }
+    SlabWindow win;
+    win.clear();
+    long long pe_fixed = 0;
+    if (active) {
+      const uint32_t i = at.slot;
+      double pe = 0.0, total[3] = {0.0, 0.0, 0.0};
+      const uint32_t cnt = B.nbr_count[i];
+      const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
+      each_entry(lst, cnt, [&](uint32_t k, uint32_t entry) {
+        double xj, yj, zj;
+        staged_pos(R, S, staged, list_index(entry, staged), B.x, B.y, B.z, xj, yj, zj);
+        const double d[3] = {xj - xi, yj - yi, zj - zi};
+        const double r2 = dist2(d[0], d[1], d[2]);
+        if (!(r2 < R.rc2)) return;
+        const double inv2 = rcp_rn(r2);  // = 1.0 / r2, correctly rounded
+        const double s6 = inv2 * inv2 * inv2;
+        double rf = s6 * __builtin_fma(48.0, s6, -24.0);
+        const double t = 4.0 * s6;
+        double u = __builtin_fma(t, s6, -t) - R.shift;
+        if (SF) {
+          const double r = __builtin_sqrt(r2);
+          rf = __builtin_fma(-r, R.force_shift, rf);
+          u = __builtin_fma(r - R.rc, R.force_shift, u);
+        }
+        pe += 0.5 * u;
+        const double q = rf / r2;
+        double w[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          w[a] = (0.5 * d[a]) * d[a] * q;
+          total[a] += w[a];
+        }
+        const double lo = zw + (d[2] < 0.0 ? d[2] : 0.0);
+        const double hi = zw + (d[2] > 0.0 ? d[2] : 0.0);
+        const double length = hi - lo;
+        long long f[3];
+        if (length < 1e-6 * width) {
+#pragma unroll
+          for (int a = 0; a < 3; a++) f[a] = to_fixed(w[a], kProfileOne);
+          win.add((int)home - hb, f, direct, home);
+          return;
+        }
+        int b = (int)floor(lo / width);
+        double left = 1.0;
+        for (uint32_t walk = 0; walk < bins + 2u; walk++) {
+          const bool last = (double)(b + 1) * width >= hi;
+          const double start = lo > (double)b * width ? lo : (double)b * width;
+          const double end = hi < (double)(b + 1) * width ? hi : (double)(b + 1) * width;
+          double share = last ? left : (end - start) / length;
+          share = share > 0.0 ? share : 0.0;
+          left -= share;
+          const uint32_t slab = (uint32_t)(((b % ibins) + ibins) % ibins);
+#pragma unroll
+          for (int a = 0; a < 3; a++) f[a] = to_fixed(w[a] * share, kProfileOne);
+          win.add(b - hb, f, direct, slab);
+          if (last) break;
+          b++;
+        }
+      });
+      pe_fixed = to_fixed(pe, kProfileOne);
+#pragma unroll
+      for (int a = 0; a < 3; a++) whole[a] += to_fixed(total[a], kProfileOne);
+    }
+    if (together) {
+      send(hmin, 7, [&](int rs) { return active && hslot == rs ? pe_fixed : 0ll; });
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        send(hmin, 4 + a, [&](int rs) {
+          long long x = 0;
+#pragma unroll
+          for (int kk = 0; kk < 3; kk++) x += active && wslot + kk == rs ? win.w[kk][a] : 0ll;
+          return x;
+        });
+    } else if (active) {
+      direct.add(home * kProfileSums + 7u, pe_fixed);
+#pragma unroll
+      for (int kk = 0; kk < 3; kk++) {
+        const uint32_t slab = (uint32_t)((((hb - 1 + kk) % ibins) + ibins) % ibins);
+#pragma unroll
+        for (int a = 0; a < 3; a++) direct.add(slab * kProfileSums + 4u + (uint32_t)a, win.w[kk][a]);
+      }
+    }
   }
 }
 
@@ -1696,11 +1915,41 @@ __device__ __forceinline__ void profile_flush(const ProfileSink& sink, uint32_t 
   }
 }
 
-template <class L, bool SF, bool PARTS>
-__global__ void __launch_bounds__(L::kThreads) k_profile(Buffers B) {
+// WIDE: every replica's slabs are at least a cutoff wide (profile_part_wide,
+// no LDS bins); its main launch is held to 128 VGPRs, so that two 512-thread
+// workgroups share a CU.
+template <class L, bool SF, bool PARTS, bool WIDE>
+__global__ void __launch_bounds__(L::kThreads, WIDE && !PARTS ? 4 : 1) k_profile(Buffers B) {
   __shared__ Stage<L> S;
-  __shared__ long long acc[kProfileLdsBins * kProfileSums + 3];
   // brick gb whole (code 0) or one part of it
+  auto profile_wide = [&](uint32_t gb, uint32_t code) {
+    const Brick brick = B.bricks[gb];
+    const Replica R = B.rep[brick.rep];
+    if (R.prof_bins == 0) return;  // the whole workgroup together
+    const Brick g = PARTS ? part_brick(brick, code, R) : brick;
+    if (PARTS) __syncthreads();  // the previous part is done with S
+    const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
+    if (!PARTS && !staged && g.gx * g.gy > 1u) return;
+    long long whole[3] = {0, 0, 0};
+    long long* prof = B.profile + R.prof_off;
+    profile_part_wide<SF>(B, R, S, g, staged, prof, whole);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const long long x = wave_sum_i64(whole[a]);
+      if ((threadIdx.x & 63u) == 0 && x != 0)
+        atomicAdd((unsigned long long*)(prof + R.prof_bins * kProfileSums + (uint32_t)a), (unsigned long long)x);
+    }
+  };
+  if constexpr (WIDE) {
+    if constexpr (!PARTS) {
+      const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
+      if (gb < layout_end<L>(B)) profile_wide(gb, 0u);
+    } else {
+      each_cut_part<L>(B, profile_wide);
+    }
+    return;
+  }
+  __shared__ long long acc[WIDE ? 1 : kProfileLdsBins * kProfileSums + 3];
   auto profile = [&](uint32_t gb, uint32_t code) {
     const Brick brick = B.bricks[gb];
     const Replica R = B.rep[brick.rep];
@@ -1718,11 +1967,13 @@ __global__ void __launch_bounds__(L::kThreads) k_profile(Buffers B) {
     profile_part<SF>(B, R, S, g, staged, sink, whole);
     profile_flush<L>(sink, bins, whole);
   };
-  if constexpr (!PARTS) {
-    const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
-    if (gb < layout_end<L>(B)) profile(gb, 0u);
-  } else {
-    each_cut_part<L>(B, profile);
+  if constexpr (!WIDE) {
+    if constexpr (!PARTS) {
+      const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
+      if (gb < layout_end<L>(B)) profile(gb, 0u);
+    } else {
+      each_cut_part<L>(B, profile);
+    }
   }
 }
 
@@ -1748,12 +1999,12 @@ __device__ __forceinline__ void heat_part(const Buffers& B, const Replica& R, co
     double pr[3] = {0.0, 0.0, 0.0};
     const uint32_t cnt = B.nbr_count[i];
     const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
-    for (uint32_t k = 0; k < cnt; k++) {
+    each_entry(lst, cnt, [&](uint32_t k, uint32_t entry) {
       double xj, yj, zj;
-      staged_pos(R, S, staged, list_index(lst[list_entry(k)], staged), B.x, B.y, B.z, xj, yj, zj);
+      staged_pos(R, S, staged, list_index(entry, staged), B.x, B.y, B.z, xj, yj, zj);
       const double d[3] = {xj - xi, yj - yi, zj - zi};
       const double r2 = dist2(d[0], d[1], d[2]);
-      if (!(r2 < R.rc2)) continue;
+      if (!(r2 < R.rc2)) return;
       const double inv2 = 1.0 / r2;
       const double s6 = inv2 * inv2 * inv2;
       double rf = s6 * __builtin_fma(48.0, s6, -24.0);
@@ -1769,7 +2020,7 @@ __device__ __forceinline__ void heat_part(const Buffers& B, const Replica& R, co
       const double q = dv * rf / r2;
 #pragma unroll
       for (int a = 0; a < 3; a++) pr[a] = __builtin_fma(d[a], q, pr[a]);
-    }
+    });
 #pragma unroll
     for (int a = 0; a < 3; a++) jsum[a] += to_fixed(__builtin_fma(e, v[a], 0.5 * pr[a]), 4294967296.0);
   }
@@ -1825,12 +2076,12 @@ __device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, 
     double v2 = 0.0;
     const uint32_t cnt = B.nbr_count[i];
     const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
-    for (uint32_t k = 0; k < cnt; k++) {
-      const uint32_t e = list_index(lst[list_entry(k)], staged);
+    each_entry(lst, cnt, [&](uint32_t k, uint32_t entry) {
+      const uint32_t e = list_index(entry, staged);
       double xj, yj, zj;
       staged_pos(R, S, staged, e, B.x, B.y, B.z, xj, yj, zj);
       const double r2 = dist2(xj - xi, yj - yi, zj - zi);
-      if (!(r2 < R.rc2)) continue;
+      if (!(r2 < R.rc2)) return;
       within++;
       near += r2 < kNearR2 ? 1u : 0u;
       if (R.drawn == 1u) {
@@ -1838,7 +2089,7 @@ __device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, 
         const uint32_t j = S.start[c] + (e - S.off[c]);
         v2 += dist2(B.vx[j], B.vy[j], B.vz[j]);
       }
-    }
+    });
     const float w = R.drawn == 1u ? (float)((dist2(v[0], v[1], v[2]) + v2) / (3.0 * (double)(within + 1u)))
                                   : (float)near;
     float4* P = reinterpret_cast<float4*>(B.render_pos);
@@ -2132,23 +2383,25 @@ hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStre
   return hipGetLastError();
 }
 
-template <bool SF>
+template <bool SF, bool WIDE>
 void profile_launches(const Buffers& B, hipStream_t s) {
   const dim3 gp = parts_grid(B);
   if (layout_count<Dense>(B))
-    hipLaunchKernelGGL((k_profile<Dense, SF, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B);
+    hipLaunchKernelGGL((k_profile<Dense, SF, false, WIDE>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B);
   if (layout_count<Sparse>(B))
-    hipLaunchKernelGGL((k_profile<Sparse, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B);
-  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_profile<Dense, SF, true>), gp, dim3(Dense::kThreads), 0, s, B);
-  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_profile<Sparse, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B);
+    hipLaunchKernelGGL((k_profile<Sparse, SF, false, WIDE>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B);
+  if (layout_count<Dense>(B))
+    hipLaunchKernelGGL((k_profile<Dense, SF, true, WIDE>), gp, dim3(Dense::kThreads), 0, s, B);
+  if (layout_count<Sparse>(B))
+    hipLaunchKernelGGL((k_profile<Sparse, SF, true, WIDE>), gp, dim3(Sparse::kThreads), 0, s, B);
 }
 
-hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s) {
+hipError_t launch_profile(const Buffers& B, bool shifted, bool wide, hipStream_t s) {
   if (!B.n_bricks) return hipSuccess;
   if (shifted)
-    profile_launches<true>(B, s);
+    wide ? profile_launches<true, true>(B, s) : profile_launches<true, false>(B, s);
   else
-    profile_launches<false>(B, s);
+    wide ? profile_launches<false, true>(B, s) : profile_launches<false, false>(B, s);
   return hipGetLastError();
 }
 

@@ -41,7 +41,8 @@ hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, 
 hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s,
                         const Side* side = nullptr);
 hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStream_t s);
-hipError_t launch_profile(const Buffers& B, bool shifted, hipStream_t s);
+// wide: every replica's profile slabs are at least a cutoff wide.
+hipError_t launch_profile(const Buffers& B, bool shifted, bool wide, hipStream_t s);
 // The float4 render arrays (positions; velocities with the drawn value in w).
 hipError_t launch_render(const Buffers& B, hipStream_t s);
 hipError_t launch_zero(long long* p, uint32_t n, hipStream_t s);

@@ -1270,7 +1270,9 @@ int vlj_sample_profile(vlj_sim* sim) {
   if (!any_profiles(sim)) return fail(sim, VLJ_E_STATE, "profiles need Settings::profile_bins above zero");
   int rc = prime_if_needed(sim);
   if (rc != VLJ_OK) return rc;
-  HIP_TRY(sim, launch_profile(sim->B, sim->shifted, sim->stream));
+  bool wide = true;
+  for (const Replica& R : sim->rep) wide = wide && (R.prof_bins == 0 || R.bin_width >= R.rc);
+  HIP_TRY(sim, launch_profile(sim->B, sim->shifted, wide, sim->stream));
   for (auto& h : sim->host)
     if (h.settings.profile_bins) h.prof_samples++;
   return VLJ_OK;
@@ -2080,6 +2082,117 @@ struct StateArrays {
   std::vector<long long> profile, hc;
 };
 
+// Everything in a state file that indexes device memory, checked on the
+// host before any device work: a file with valid sizes but bad contents
+// gets VLJ_E_INVALID instead of driving out-of-bounds accesses. Returns ""
+// or what is wrong.
+std::string check_state(const std::vector<Replica>& rep, const std::vector<Brick>& bricks, const StateHeader& h,
+                        const std::vector<Control>& ctl, const StateArrays& a) {
+  const size_t R = rep.size(), nb = bricks.size();
+  for (size_t k = 0; k < R; k++) {
+    const Replica& x = rep[k];
+    if (x.cells[0] < 2 || x.cells[1] < 2 || x.cells[2] < 2 ||
+        (uint64_t)x.cells[0] * x.cells[1] * x.cells[2] != x.total_cells || x.total_cells > (1u << 20))
+      return "replica " + std::to_string(k) + ": bad cell grid";
+    if (x.capacity == 0 || (k > 0 && x.hc_cap != 0) || (k == 0 && x.hc_cap != h.hc_cap))
+      return "replica " + std::to_string(k) + ": bad capacity or heat-current series";
+    if (x.prof_bins > kMaxProfileBins) return "replica " + std::to_string(k) + ": bad profile slabs";
+    if (ctl[k].rebuild > 1u || ctl[k].reorder > 1u) return "replica " + std::to_string(k) + ": bad control words";
+  }
+  // bricks: inside their replica's rows, tiling every replica's cells once
+  std::vector<uint32_t> owner(h.cells_total, 0xffffffffu);
+  for (size_t b = 0; b < nb; b++) {
+    const Brick& g = bricks[b];
+    if (g.rep >= R || g.gx < 1 || g.gy < 1 || g.gx > (uint32_t)kBrickMax || g.gy > (uint32_t)kBrickMax)
+      return "bad brick table";
+    const Replica& x = rep[g.rep];
+    if (g.cell0 >= x.total_cells) return "bad brick table";
+    const uint32_t cx = g.cell0 % x.cells[0], cy = (g.cell0 / x.cells[0]) % x.cells[1];
+    if (cx + g.gx > x.cells[0] || cy + g.gy > x.cells[1]) return "a brick runs past its row";
+    for (uint32_t yy = 0; yy < g.gy; yy++)
+      for (uint32_t xx = 0; xx < g.gx; xx++) {
+        uint32_t& o = owner[x.cell_off + g.cell0 + yy * x.cells[0] + xx];
+        if (o != 0xffffffffu) return "bricks overlap";
+        o = (uint32_t)b;
+      }
+  }
+  for (uint32_t o : owner)
+    if (o == 0xffffffffu) return "bricks do not cover every cell";
+  // parts of cut bricks
+  const size_t ng = (nb + kCutGroup - 1) / kCutGroup;
+  std::vector<uint32_t> group_cut(ng, 0);
+  for (size_t b = 0; b < nb; b++) {
+    const uint32_t np = a.cut[b];
+    if (np > (uint32_t)kHomeCells) return "bad part count";
+    if (np) group_cut[b / kCutGroup]++;
+    for (uint32_t u = 0; u < np; u++) {
+      const uint32_t c = a.parts[b * kHomeCells + u];
+      const uint32_t x0 = c & 15u, y0 = (c >> 4) & 15u, w = (c >> 8) & 15u, hh = c >> 12;
+      if (w < 1 || hh < 1 || x0 + w > bricks[b].gx || y0 + hh > bricks[b].gy) return "bad part of a cut brick";
+    }
+  }
+  for (size_t g = 0; g < ng; g++)
+    if (a.cut_groups[g] != group_cut[g]) return "cut counts do not match the cut bricks";
+  // cells in cell order inside their replica; atoms and lists inside theirs
+  // (before the first list build the cell tables and lists hold nothing)
+  if (!h.primed) return "";
+  for (size_t k = 0; k < R; k++) {
+    const Replica& x = rep[k];
+    uint32_t next = x.atom_off;
+    for (uint32_t c = x.cell_off; c < x.cell_off + x.total_cells; c++) {
+      if (a.cell_start[c] != next || a.cell_n[c] > x.count) return "bad cell table";
+      next += a.cell_n[c];
+    }
+    if (next != x.atom_off + x.count) return "bad cell table";
+    std::vector<char> seen(x.count, 0);
+    for (uint32_t i = x.atom_off; i < x.atom_off + x.count; i++) {
+      if (a.orig[i] >= x.count || seen[a.orig[i]]) return "bad original index";
+      seen[a.orig[i]] = 1;
+      if (a.cell_of[i] < x.cell_off || a.cell_of[i] >= x.cell_off + x.total_cells) return "bad cell of an atom";
+      if (a.nbr_count[i] > x.capacity) return "a list longer than the capacity";
+    }
+  }
+  // list entries: staged indices below the staged count of the atom's brick
+  // (or part), from the same cell counts the kernels stage
+  for (size_t b = 0; b < nb; b++) {
+    const Brick& whole = bricks[b];
+    const Replica& x = rep[whole.rep];
+    const uint32_t np = a.cut[b];
+    const uint32_t stage_max = b < h.n_dense ? (uint32_t)kStageMax : (uint32_t)Sparse::kStageMax;
+    for (uint32_t u = 0; u < (np ? np : 1u); u++) {
+      Brick g = whole;
+      if (np) {
+        const uint32_t c = a.parts[b * kHomeCells + u];
+        g = Brick{whole.rep, whole.cell0 + ((c >> 4) & 15u) * x.cells[0] + (c & 15u), (c >> 8) & 15u, c >> 12};
+      }
+      const int nx = (int)x.cells[0], ny = (int)x.cells[1], nz = (int)x.cells[2];
+      const int cx0 = (int)(g.cell0 % x.cells[0]), cy0 = (int)((g.cell0 / x.cells[0]) % x.cells[1]);
+      const int cz = (int)(g.cell0 / (x.cells[0] * x.cells[1]));
+      uint64_t total = 0;
+      for (int dz = 0; dz < 3; dz++)
+        for (int yy = 0; yy < (int)g.gy + 2; yy++)
+          for (int xx = 0; xx < (int)g.gx + 2; xx++) {
+            const int ccx = (cx0 - 1 + xx + nx) % nx, ccy = (cy0 - 1 + yy + ny) % ny, ccz = (cz - 1 + dz + nz) % nz;
+            total += a.cell_n[x.cell_off + (uint32_t)((ccz * ny + ccy) * nx + ccx)];
+          }
+      const bool staged = total <= stage_max;
+      for (uint32_t yy = 0; yy < g.gy; yy++)
+        for (uint32_t xx = 0; xx < g.gx; xx++) {
+          const uint32_t c = x.cell_off + g.cell0 + yy * x.cells[0] + xx;
+          for (uint32_t i = a.cell_start[c]; i < a.cell_start[c] + a.cell_n[c]; i++) {
+            const uint64_t base = list_base(x, i - x.atom_off);
+            for (uint32_t k = 0; k < a.nbr_count[i]; k++) {
+              const uint64_t at = base + list_entry(k);
+              if (at >= h.list_total) return "a list outside the list memory";
+              if (list_index(a.nbr[at], staged) >= total) return "a list entry outside its staged neighbourhood";
+            }
+          }
+        }
+    }
+  }
+  return "";
+}
+
 double* const* state_f64(Buffers& B) {
   static thread_local double* p[12];
   double* q[12] = {B.x, B.y, B.z, B.vx, B.vy, B.vz, B.ax, B.ay, B.az, B.xb, B.yb, B.zb};
@@ -2258,10 +2371,10 @@ int vlj_load(const char* path, vlj_sim** out) {
        read_vec(r, a.profile, h.prof_total) && read_vec(r, a.hc, (size_t)h.hc_cap * 3);
   if (!ok) return bad("truncated");
   std::fclose(r.f);
-  for (const Brick& g : bricks)
-    if (g.rep >= R || g.gx < 1 || g.gy < 1 || g.gx > (uint32_t)kBrickMax || g.gy > (uint32_t)kBrickMax ||
-        g.cell0 >= rep[g.rep].total_cells)
-      return fail(nullptr, VLJ_E_INVALID, std::string(path) + ": bad brick table");
+  {
+    std::string why = check_state(rep, bricks, h, ctl, a);
+    if (!why.empty()) return fail(nullptr, VLJ_E_INVALID, std::string(path) + ": " + why);
+  }
   // The particles in original order, to build the handle from.
   std::vector<vlj_particle> parts(N);
   std::vector<uint64_t> offsets(R + 1, 0);
@@ -2294,7 +2407,9 @@ int vlj_load(const char* path, vlj_sim** out) {
     if (x.count != y.count || x.total_cells != y.total_cells || x.capacity != y.capacity ||
         x.list_off != y.list_off || x.prof_off != y.prof_off || x.cells[0] != y.cells[0] ||
         x.cells[1] != y.cells[1] || x.cells[2] != y.cells[2] || x.atom_block_off != y.atom_block_off ||
-        x.scan_block_off != y.scan_block_off)
+        x.scan_block_off != y.scan_block_off || x.atom_off != y.atom_off || x.cell_off != y.cell_off ||
+        x.scan_blocks != y.scan_blocks || x.scan_cells != y.scan_cells || x.atom_blocks != y.atom_blocks ||
+        x.prof_bins != y.prof_bins)
       return reject(VLJ_E_INVALID, "replica " + std::to_string(k) + ": the settings give another layout than the saved one");
   }
   // The saved bricks replace the ones planned from the particles.
