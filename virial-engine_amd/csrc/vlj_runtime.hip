@@ -409,10 +409,10 @@ int capture(vlj_sim* s, uint32_t steps, hipGraphExec_t* out) {
 // kBrickMax each way) whose every brick stages at most `fill` of kStageMax
 // atoms. The count a brick is held to is the larger of what it stages at
 // its own layer and at the layers above and below (a density profile may
-// drift by a cell, as a liquid-vapour interface does) and the replica's
-// mean occupancy (a lattice that is not yet uniform evens out). The
-// occupancy is the initial state's when a handle is created and that of
-// the last list build when a replica is planned again. Window sums come
+// drift by a cell, as a liquid-vapour interface does) and what it would
+// stage at the replica's mean occupancy. The occupancy is the initial
+// state's when a handle is created and that of the last list build when a
+// replica is planned again. Window sums come
 // from a 2D prefix sum of the three layers' counts over the periodically
 // extended plane. A brick that outgrows LDS later on is cut into parts on
 // the device. A layer whose bricks are held to at most `sparse_fill` of the
@@ -425,7 +425,21 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fi
 #endif
   const double limit = fill * kStageMax;
   const double sparse_limit = sparse_fill * Sparse::kStageMax;
-  const double mean = VLJ_MEAN_FLOOR * (double)R.count / (double)R.total_cells;
+  // Two floors for a brick's count: the replica's mean occupancy (a
+  // lattice filled from the bottom, with empty layers above, melts into
+  // them; a gas condenses into droplets anywhere) sets the brick shape; the
+  // mean of the layer and its neighbours (a slab's vapour stays a vapour)
+  // sets the layout.
+  const double whole = VLJ_MEAN_FLOOR * (double)R.count / (double)R.total_cells;
+  std::vector<double> layer_mean(nz, 0.0);
+  {
+    std::vector<uint64_t> layer(nz, 0);
+    for (int z = 0; z < nz; z++)
+      for (size_t c = 0; c < (size_t)nx * ny; c++) layer[z] += occ[(size_t)z * nx * ny + c];
+    for (int z = 0; z < nz; z++)
+      layer_mean[z] = VLJ_MEAN_FLOOR *
+                      (double)(layer[(z + nz - 1) % nz] + layer[z] + layer[(z + 1) % nz]) / (3.0 * nx * ny);
+  }
   const int ex = nx + 2, ey = ny + 2;  // plane extended by one cell each side
   const size_t plane = (size_t)(ex + 1) * (ey + 1);
   std::vector<uint64_t> pre(plane * nz);
@@ -460,27 +474,33 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fi
   sparse.assign(nz, 0);
   for (int z = 0; z < nz; z++) {
     uint32_t best = 0;
-    double best_worst = limit;
+    double best_worst = limit, best_local = limit;
     for (int gy = 1; gy <= std::min(kBrickMax, ny); gy++)
       for (int gx = gy; gx <= std::min(kBrickMax, nx); gx++) {
         const uint32_t home = (uint32_t)(gx * gy);
         if (home <= best) continue;
-        double worst = margin((uint64_t)(mean * 3.0 * (gx + 2) * (gy + 2)));
-        for (int dz = -1; dz <= 1 && worst <= limit; dz++) {
+        const double staged_cells = 3.0 * (gx + 2) * (gy + 2);
+        double windows = 0.0;  // the largest window, with its margin
+        const double floor_count = margin((uint64_t)(whole * staged_cells));
+        for (int dz = -1; dz <= 1 && windows <= limit; dz++) {
           const int zz = (z + dz + nz) % nz;
-          for (int y0 = 0; y0 < ny && worst <= limit; y0 += gy)
-            for (int x0 = 0; x0 < nx && worst <= limit; x0 += gx)
-              worst = std::max(worst, margin(window(zz, x0, y0, std::min(gx, nx - x0), std::min(gy, ny - y0))));
+          for (int y0 = 0; y0 < ny && windows <= limit; y0 += gy)
+            for (int x0 = 0; x0 < nx && windows <= limit; x0 += gx)
+              windows = std::max(windows, margin(window(zz, x0, y0, std::min(gx, nx - x0), std::min(gy, ny - y0))));
         }
+        const double worst = std::max(windows, floor_count);
         if (worst <= limit) {
           best = home;
           best_worst = worst;
+          best_local = std::max(windows, margin((uint64_t)(layer_mean[z] * staged_cells)));
           bx[z] = (uint32_t)gx;
           by[z] = (uint32_t)gy;
         }
       }
-    // a layer whose bricks stage few atoms takes the sparse layout
-    sparse[z] = best > 0 && best_worst <= sparse_limit ? 1 : 0;
+    // a layer whose bricks stage few atoms takes the sparse layout: few by
+    // its own counts, and no more than twice the sparse limit by the
+    // replica's mean
+    sparse[z] = best > 0 && best_local <= sparse_limit && best_worst <= 2.0 * sparse_limit ? 1 : 0;
   }
 }
 
