@@ -1129,9 +1129,14 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
             const unsigned long long mask =
                 __builtin_amdgcn_fcmp(d2, R.list_r2, 4) & __builtin_amdgcn_uicmp(q[c], e, 33);
             const uint32_t pos = f + lane_rank(mask);
-            if (near && pos < cap)
-              *(uint16_t*)(lst + (size_t)(((pos & ~7u) << 7) | ((pos & 7u) << 1))) = (uint16_t)qe[c];
-            f += (uint32_t)__builtin_popcountll(mask);
+            const uint32_t hits = (uint32_t)__builtin_popcountll(mask);
+            // past the capacity only while the list fills its last entries
+            // (a scalar test, so the usual chunk tests no lane against it)
+            bool store = near;
+            if (__builtin_expect(f + hits > cap, 0)) store = near && pos < cap;
+            // byte offset (pos / 8) KB + 2 (pos % 8) = 2 pos + 1008 (pos / 8)
+            if (store) *(uint16_t*)(lst + (size_t)(__umul24(pos >> 3, 1008u) + 2u * pos)) = (uint16_t)qe[c];
+            f += hits;
           }
         }
         found = lane == k - k0 ? f : found;
