@@ -338,19 +338,40 @@ def counter_means(path):
     return {k: {cn: sum(v) / len(v) for cn, v in cs.items()} for k, cs in d.items()}
 
 
+def working_means(path):
+    """Counter means over each kernel's working dispatches: those whose busy
+    cycles (or, without that counter, whose sum of all counters) reach half
+    of the kernel's largest. The list build and the rebuild chain return at
+    once on steps without a rebuild; their plain means mix both."""
+    c = _db(path)
+    per = defaultdict(lambda: defaultdict(dict))
+    for did, k, cn, v in c.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+        per[_short(k)][did][cn] = v
+    out = {}
+    for k, ds in per.items():
+        key = lambda r: r.get("SQ_BUSY_CYCLES", sum(r.values()))
+        top = max(key(r) for r in ds.values())
+        work = [r for r in ds.values() if key(r) >= 0.5 * top]
+        out[k] = (len(work), len(ds), {cn: sum(r[cn] for r in work) / len(work) for cn in work[0]})
+    return out
+
+
 def summarize(path, out, command):
     lines = [f"# rocprofv3 summary of: {command}", ""]
     c = _db(path)
     has_counters = c.execute("select count(*) from sqlite_master where name = 'counters_collection'").fetchone()[0]
     if has_counters and c.execute("select count(*) from counters_collection").fetchone()[0]:
-        lines.append(f"{'kernel':44s} counter means per dispatch")
+        lines.append(f"{'kernel':60s} counter means per dispatch")
         for k, cs in sorted(counter_means(path).items()):
-            lines.append(f"{k[:44]:44s} " + ", ".join(f"{cn}={v:.1f}" for cn, v in sorted(cs.items())))
+            lines.append(f"{k[:60]:60s} " + ", ".join(f"{cn}={v:.1f}" for cn, v in sorted(cs.items())))
+        lines += ["", f"{'kernel':60s} counter means over the working dispatches (at least half the largest)"]
+        for k, (n, of, cs) in sorted(working_means(path).items()):
+            lines.append(f"{k[:60]:60s} [{n}/{of}] " + ", ".join(f"{cn}={v:.1f}" for cn, v in sorted(cs.items())))
     else:
         stats = kernel_stats(path)
-        lines.append(f"{'kernel':44s} {'calls':>6s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} {'total_ms':>10s}")
+        lines.append(f"{'kernel':60s} {'calls':>6s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} {'total_ms':>10s}")
         for k, v in sorted(stats.items(), key=lambda kv: -sum(kv[1])):
-            lines.append(f"{k[:44]:44s} {len(v):6d} {sum(v) / len(v):10.2f} {min(v):10.2f} {max(v):10.2f} "
+            lines.append(f"{k[:60]:60s} {len(v):6d} {sum(v) / len(v):10.2f} {min(v):10.2f} {max(v):10.2f} "
                          f"{sum(v) / 1e3:10.3f}")
     open(out, "w").write("\n".join(lines) + "\n")
     print("\n".join(lines))

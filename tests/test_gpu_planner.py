@@ -69,7 +69,7 @@ def test_cut_bricks_are_planned_again(oracle, lj):
     sim.run(600)
     c0 = sim.debug_counters()
     assert c0["stage_overflow_builds"] > 0
-    sim.run(600)  # the check of step 512 arrives and plans again
+    sim.run(600)  # a check (every 256 steps) arrives and plans again
     info = sim.plan_info()
     c1 = sim.debug_counters()
     assert info["replans"] >= 1

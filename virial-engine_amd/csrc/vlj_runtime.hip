@@ -28,16 +28,14 @@ constexpr uint32_t kGraphSteps = 16;
 // (kTuneSteps steps each) on the handle's own state; a handle of at least
 // kTuneMinAtoms atoms tunes itself on the first call that asks for at least
 // kTuneAutoSteps steps. Every kCheckSteps steps the control words are copied
-// back without a wait, and a replica whose bricks were cut at more than
-// kReplanCutShare of its builds' bricks is planned again from its present
-// cell occupancy.
+// back without a wait, and a replica whose bricks were cut since the
+// check before is planned again from its present cell occupancy.
 constexpr double kDefaultFill = 0.9;
 constexpr double kTuneFills[] = {0.8, 0.9, 1.0};
 constexpr uint32_t kTuneSteps = 6;
 constexpr uint32_t kTuneMinAtoms = 65536;
 constexpr uint64_t kTuneAutoSteps = 64;
-constexpr uint64_t kCheckSteps = 512;
-constexpr double kReplanCutShare = 0.02;
+constexpr uint64_t kCheckSteps = 256;
 constexpr double kTwo32 = 4294967296.0;
 constexpr double kPi = 3.14159265358979323846;  // the f64 nearest pi (std::f64::consts::PI)
 
@@ -704,8 +702,8 @@ bool tune_wanted(const vlj_sim* s) {
 
 // The periodic check: takes the control words copied back at the last
 // check, if they have arrived, and plans again every replica whose bricks
-// were cut at more than kReplanCutShare of its bricks per list build since
-// the check before (when that changes its plan); then starts the next copy
+// were cut since the check before (when that changes its plan); then
+// starts the next copy
 // once kCheckSteps steps have passed. Never waits for the device except
 // to plan again.
 int check_bricks(vlj_sim* s) {
@@ -729,7 +727,7 @@ int check_bricks(vlj_sim* s) {
       const uint32_t dcut = c.stage_overflow - s->cut_seen[r], dreb = c.rebuilds - s->rebuilds_seen[r];
       s->cut_seen[r] = c.stage_overflow;
       s->rebuilds_seen[r] = c.rebuilds;
-      if (dreb == 0 || (double)dcut <= kReplanCutShare * s->rep_bricks[r] * dreb) continue;
+      if (dreb == 0 || dcut == 0) continue;
       // plan it from the present occupancy only if that gives other bricks
       which[r] = 1;
       any = true;
