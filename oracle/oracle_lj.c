@@ -1242,17 +1242,39 @@ static void pair_terms(const vo_params *p, double r2, double *rf_out, double *u_
 
 /* R10 profiles along z (lennard_jones.wgsl:628-730): per slab the count,
  * m v_a^2, the Irving-Kirkwood virial diagonal W_aa and the potential, then
- * the whole-box W_aa. Each (pair, slab, axis) contribution and each per-atom
- * value goes to fixed point separately, llrint(x 2^32), as the reference
- * adds each with fixed_add; the integers are summed. A pair's half of the
- * virial, w_a = (0.5 d_a) d_a (rf / r2), is spread over the slabs the
- * segment from z_i to z_i + d_z crosses, in proportion to the length inside
- * each, the last slab taking what is left. Slabs use the wrapped z. The
- * pair terms are FP64 in both precision forms. The walk over slabs stops
- * after bins + 2 slabs, more than a segment shorter than the box can cross. */
+ * the whole-box W_aa. A pair's half of the virial, w_a = (0.5 d_a) d_a q with
+ * q = rf (1 / r2), is spread over the slabs the segment from z_i to z_i + d_z
+ * crosses, in proportion to the length inside each, the last slab taking
+ * what is left, starting from slab floor(lo iw), lo the segment's lower end
+ * and iw = 1 / width. Slabs use the wrapped z; hb = floor(z_i iw) is atom
+ * i's slab. Atom i's shares to the slabs hb - 1, hb and hb + 1 (every share,
+ * when slabs are at least a cutoff wide) are summed per slab and axis in
+ * FP64, in list order, and each of the nine sums goes to fixed point once,
+ * llrint(x 2^32); a share to any other slab goes to fixed point on its own.
+ * Each per-atom value (count, v_a^2, potential, whole-box W_aa) goes to fixed
+ * point on its own, and the integers are summed, so the totals do not depend
+ * on the order of atoms (the reference adds each contribution with
+ * fixed_add). The pair terms are FP64 in both precision forms. The walk over
+ * slabs stops after bins + 2 slabs, more than a segment shorter than the box
+ * can cross. */
 static inline void prof_add(int64_t *prof, uint32_t slab, int q, double x)
 {
     prof[slab * 8u + (uint32_t)q] += llrint(x * TWO32);
+}
+
+/* Share `share` of w (one value per axis) to the slab k - hb = rel from atom
+ * i's: into the atom's FP64 window sums when rel is -1, 0 or 1, else to
+ * fixed point on its own. */
+static inline void prof_share(int64_t *prof, double win[3][3], int rel, uint32_t slab, const double w[3],
+                              double share)
+{
+    for (int a = 0; a < 3; a++) {
+        const double f = w[a] * share;
+        if (rel >= -1 && rel <= 1)
+            win[rel + 1][a] += f;
+        else
+            prof_add(prof, slab, 4 + a, f);
+    }
 }
 
 void vo_set_profile_bins(vo_sim *s, uint32_t bins)
@@ -1279,15 +1301,17 @@ void vo_sample_profile(vo_sim *s)
         return;
     int64_t *prof = s->prof;
     const double width = p->box[2] / (double)bins;
+    const double iw = 1.0 / width;
     for (uint32_t i = 0; i < p->count; i++) {
         const double xi = s->x[0][i], yi = s->x[1][i], zraw = s->x[2][i];
         const double zi = wrap_step(zraw, p->box[2]);
-        int hb = (int)floor(zi / width);
+        int hb = (int)floor(zi * iw);
         const uint32_t home = (uint32_t)(hb < 0 ? 0 : (hb > (int)bins - 1 ? (int)bins - 1 : hb));
         prof_add(prof, home, 0, 1.0);
         for (int k = 0; k < 3; k++)
             prof_add(prof, home, 1 + k, s->v[k][i] * s->v[k][i]);
         double pe = 0.0, total[3] = {0.0, 0.0, 0.0};
+        double win[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}}; /* slabs hb - 1, hb, hb + 1 */
         const uint32_t cnt = s->nbr_count[i];
         const uint32_t *row = s->nbr + (size_t)i * p->capacity;
         const uint8_t *img = s->nbr_image + (size_t)i * p->capacity;
@@ -1303,7 +1327,7 @@ void vo_sample_profile(vo_sim *s)
             double rf, u;
             pair_terms(p, r2, &rf, &u);
             pe += 0.5 * u;
-            const double q = rf / r2;
+            const double q = rf * (1.0 / r2);
             double w[3];
             for (int a = 0; a < 3; a++) {
                 w[a] = (0.5 * d[a]) * d[a] * q;
@@ -1313,11 +1337,10 @@ void vo_sample_profile(vo_sim *s)
             const double hi = zi + (d[2] > 0.0 ? d[2] : 0.0);
             const double length = hi - lo;
             if (length < 1e-6 * width) {
-                for (int a = 0; a < 3; a++)
-                    prof_add(prof, home, 4 + a, w[a]);
+                prof_share(prof, win, (int)home - hb, home, w, 1.0);
                 continue;
             }
-            int b = (int)floor(lo / width);
+            int b = (int)floor(lo * iw);
             double left = 1.0;
             for (uint32_t walk = 0; walk < bins + 2u; walk++) {
                 const int last = (double)(b + 1) * width >= hi;
@@ -1327,12 +1350,16 @@ void vo_sample_profile(vo_sim *s)
                 share = share > 0.0 ? share : 0.0;
                 left -= share;
                 const uint32_t slab = (uint32_t)(((b % (int)bins) + (int)bins) % (int)bins);
-                for (int a = 0; a < 3; a++)
-                    prof_add(prof, slab, 4 + a, w[a] * share);
+                prof_share(prof, win, b - hb, slab, w, share);
                 if (last)
                     break;
                 b++;
             }
+        }
+        for (int kk = 0; kk < 3; kk++) {
+            const uint32_t slab = (uint32_t)((((hb - 1 + kk) % (int)bins) + (int)bins) % (int)bins);
+            for (int a = 0; a < 3; a++)
+                prof_add(prof, slab, 4 + a, win[kk][a]);
         }
         prof_add(prof, home, 7, pe);
         for (int a = 0; a < 3; a++)

@@ -1603,15 +1603,18 @@ __global__ void __launch_bounds__(L::kThreads) k_measure(Buffers B) {
 // sums from b * kProfileSums: particle count, v_a^2 (a = x, y, z), the
 // Irving-Kirkwood W_aa, potential energy; after the slabs, the whole-box
 // W_aa. Counts, kinetic terms and potential go to the slab of the atom's
-// wrapped z. A pair's half of the virial, w_a = (0.5 d_a) d_a (rf / r2), is
-// spread over the slabs that the segment from z_i to z_i + d_z crosses, in
-// proportion to the length inside each, the last slab taking what is left.
-// Every (pair, slab, axis) share and every per-atom value goes to fixed point
-// on its own as rint(x 2^32) and the integers are added, so the sums do not
-// depend on the order of the additions. Pair terms are FP64 in both precision
-// forms. Up to kProfileLdsBins (256) slabs a workgroup adds into LDS and then once
-// per touched sum into the replica's sums; with more slabs it adds straight
-// into them.
+// wrapped z. A pair's half of the virial, w_a = (0.5 d_a) d_a q with
+// q = rf (1 / r2), is spread over the slabs that the segment from z_i to
+// z_i + d_z crosses, in proportion to the length inside each, the last slab
+// taking what is left; the slab of z is floor(z (1 / width)). An atom's
+// shares to its own slab and the slabs beside it are summed in FP64 in list
+// order (SlabWindow) and go to fixed point once per atom, slab and axis; a
+// share to a slab farther away (slabs narrower than the cutoff) and every
+// per-atom value go to fixed point on their own, as rint(x 2^32), and the
+// integers are added, so the sums do not depend on the order of the atoms.
+// Pair terms are FP64 in both precision forms. Up to kProfileLdsBins (256)
+// slabs a workgroup adds into LDS and then once per touched sum into the
+// replica's sums; with more slabs it adds straight into them.
 // ---------------------------------------------------------------------------
 
 constexpr uint32_t kProfileLdsBins = 256;
@@ -1632,29 +1635,36 @@ struct ProfileSink {
 };
 
 // The virial shares of one atom's pairs to the slab below its own slab,
-// its own and the one above, each share already in fixed point: added up
-// in registers (integer sums, so the order does not matter) and sent to the
-// profile once per atom instead of once per share. With slabs at least a
-// cutoff wide every share lands in these three; a share elsewhere (narrow
-// slabs) goes to the profile at once.
+// its own and the one above, summed in FP64 in list order and sent to the
+// profile in fixed point once per atom. With slabs at least a cutoff wide
+// every share lands in these three; a share elsewhere (narrow slabs) goes
+// to the profile at once.
 struct SlabWindow {
-  long long w[3][3];  // [slab below, own, above][axis]
+  double w[3][3];  // [slab below, own, above][axis]
   __device__ __forceinline__ void clear() {
 #pragma unroll
     for (int k = 0; k < 3; k++)
 #pragma unroll
-      for (int a = 0; a < 3; a++) w[k][a] = 0;
+      for (int a = 0; a < 3; a++) w[k][a] = 0.0;
   }
-  // k: the share's slab relative to the atom's (-1, 0, 1 fit)
-  __device__ __forceinline__ void add(int k, const long long f[3], const ProfileSink& sink, uint32_t slab) {
+  // w_a share to slab b (unwrapped), k = b - hb from the atom's (-1, 0, 1
+  // fit). Each sum takes fma(f, 1, sum) = sum + f or fma(f, 0, sum) = sum:
+  // one instruction per sum and no select (a non-finite share, from atoms
+  // on top of each other, spoils all three of the atom's sums instead of
+  // one).
+  __device__ __forceinline__ void add(int k, int b, int bins, const double wv[3], double share,
+                                      const ProfileSink& sink) {
     if (k >= -1 && k <= 1) {
 #pragma unroll
-      for (int kk = 0; kk < 3; kk++)
+      for (int kk = 0; kk < 3; kk++) {
+        const double sel = kk == k + 1 ? 1.0 : 0.0;
 #pragma unroll
-        for (int a = 0; a < 3; a++) w[kk][a] += kk == k + 1 ? f[a] : 0ll;
+        for (int a = 0; a < 3; a++) w[kk][a] = __builtin_fma(wv[a] * share, sel, w[kk][a]);
+      }
     } else {
+      const uint32_t slab = (uint32_t)(((b % bins) + bins) % bins);
 #pragma unroll
-      for (int a = 0; a < 3; a++) sink.add(slab * kProfileSums + 4u + (uint32_t)a, f[a]);
+      for (int a = 0; a < 3; a++) sink.add(slab * kProfileSums + 4u + (uint32_t)a, wv[a] * share);
     }
   }
 };
@@ -1664,6 +1674,7 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
                                              bool staged, const ProfileSink& sink, long long whole[3]) {
   const uint32_t bins = R.prof_bins;
   const double width = R.bin_width;
+  const double iw = 1.0 / width;  // slab of z: floor(z iw)
   const int ibins = (int)bins;
   const uint32_t H = S.hoff[g.gx * g.gy];
   for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
@@ -1672,7 +1683,7 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
     double xi, yi, zi;
     staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
     const double zw = wrap_one(zi, R.box[2]);
-    const int hb = (int)floor(zw / width);  // the slab the shares are counted from
+    const int hb = (int)floor(zw * iw);  // the slab the shares are counted from
     const uint32_t home = (uint32_t)(hb < 0 ? 0 : (hb > ibins - 1 ? ibins - 1 : hb));
     sink.add(home * kProfileSums, 1.0);
     const double v[3] = {B.vx[i], B.vy[i], B.vz[i]};
@@ -1689,7 +1700,7 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
       const double d[3] = {xj - xi, yj - yi, zj - zi};
       const double r2 = dist2(d[0], d[1], d[2]);
       if (!(r2 < R.rc2)) return;
-      const double inv2 = 1.0 / r2;
+      const double inv2 = rcp_rn(r2);  // = 1.0 / r2, correctly rounded
       const double s6 = inv2 * inv2 * inv2;
       double rf = s6 * __builtin_fma(48.0, s6, -24.0);
       const double t = 4.0 * s6;
@@ -1700,7 +1711,7 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
         u = __builtin_fma(r - R.rc, R.force_shift, u);
       }
       pe += 0.5 * u;
-      const double q = rf / r2;
+      const double q = rf * inv2;
       double w[3];
 #pragma unroll
       for (int a = 0; a < 3; a++) {
@@ -1710,14 +1721,11 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
       const double lo = zw + (d[2] < 0.0 ? d[2] : 0.0);
       const double hi = zw + (d[2] > 0.0 ? d[2] : 0.0);
       const double length = hi - lo;
-      long long f[3];
       if (length < 1e-6 * width) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) f[a] = to_fixed(w[a], kProfileOne);
-        win.add((int)home - hb, f, sink, home);
+        win.add((int)home - hb, (int)home, ibins, w, 1.0, sink);
         return;
       }
-      int b = (int)floor(lo / width);
+      int b = (int)floor(lo * iw);
       double left = 1.0;
       // A segment shorter than the box crosses at most bins + 1 slabs.
       for (uint32_t walk = 0; walk < bins + 2u; walk++) {
@@ -1727,10 +1735,7 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
         double share = last ? left : (end - start) / length;
         share = share > 0.0 ? share : 0.0;
         left -= share;
-        const uint32_t slab = (uint32_t)(((b % ibins) + ibins) % ibins);
-#pragma unroll
-        for (int a = 0; a < 3; a++) f[a] = to_fixed(w[a] * share, kProfileOne);
-        win.add(b - hb, f, sink, slab);
+        win.add(b - hb, b, ibins, w, share, sink);
         if (last) break;
         b++;
       }
@@ -1771,6 +1776,7 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
                                                   const Brick& g, bool staged, long long* prof, long long whole[3]) {
   const uint32_t bins = R.prof_bins;
   const double width = R.bin_width;
+  const double iw = 1.0 / width;  // slab of z: floor(z iw)
   const int ibins = (int)bins;
   const ProfileSink direct{nullptr, prof, false};
   const uint32_t H = S.hoff[g.gx * g.gy];
@@ -1797,7 +1803,7 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
       at = home_atom(g, S, h);
       staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
       zw = wrap_one(zi, R.box[2]);
-      hb = (int)floor(zw / width);
+      hb = (int)floor(zw * iw);
       home = (uint32_t)(hb < 0 ? 0 : (hb > ibins - 1 ? ibins - 1 : hb));
     }
     const int hmin = wave_min_i32(hb);
@@ -1847,7 +1853,7 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
           u = __builtin_fma(r - R.rc, R.force_shift, u);
         }
         pe += 0.5 * u;
-        const double q = rf / r2;
+        const double q = rf * inv2;
         double w[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -1857,14 +1863,11 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
         const double lo = zw + (d[2] < 0.0 ? d[2] : 0.0);
         const double hi = zw + (d[2] > 0.0 ? d[2] : 0.0);
         const double length = hi - lo;
-        long long f[3];
         if (length < 1e-6 * width) {
-#pragma unroll
-          for (int a = 0; a < 3; a++) f[a] = to_fixed(w[a], kProfileOne);
-          win.add((int)home - hb, f, direct, home);
+          win.add((int)home - hb, (int)home, ibins, w, 1.0, direct);
           return;
         }
-        int b = (int)floor(lo / width);
+        int b = (int)floor(lo * iw);
         double left = 1.0;
         for (uint32_t walk = 0; walk < bins + 2u; walk++) {
           const bool last = (double)(b + 1) * width >= hi;
@@ -1873,10 +1876,7 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
           double share = last ? left : (end - start) / length;
           share = share > 0.0 ? share : 0.0;
           left -= share;
-          const uint32_t slab = (uint32_t)(((b % ibins) + ibins) % ibins);
-#pragma unroll
-          for (int a = 0; a < 3; a++) f[a] = to_fixed(w[a] * share, kProfileOne);
-          win.add(b - hb, f, direct, slab);
+          win.add(b - hb, b, ibins, w, share, direct);
           if (last) break;
           b++;
         }
@@ -1885,6 +1885,11 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
 #pragma unroll
       for (int a = 0; a < 3; a++) whole[a] += to_fixed(total[a], kProfileOne);
     }
+    long long wf[3][3];  // the window's sums in fixed point
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) wf[kk][a] = active ? to_fixed(win.w[kk][a], kProfileOne) : 0ll;
     if (together) {
       send(hmin, 7, [&](int rs) { return active && hslot == rs ? pe_fixed : 0ll; });
 #pragma unroll
@@ -1892,7 +1897,7 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
         send(hmin, 4 + a, [&](int rs) {
           long long x = 0;
 #pragma unroll
-          for (int kk = 0; kk < 3; kk++) x += active && wslot + kk == rs ? win.w[kk][a] : 0ll;
+          for (int kk = 0; kk < 3; kk++) x += active && wslot + kk == rs ? wf[kk][a] : 0ll;
           return x;
         });
     } else if (active) {
@@ -1901,7 +1906,7 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
       for (int kk = 0; kk < 3; kk++) {
         const uint32_t slab = (uint32_t)((((hb - 1 + kk) % ibins) + ibins) % ibins);
 #pragma unroll
-        for (int a = 0; a < 3; a++) direct.add(slab * kProfileSums + 4u + (uint32_t)a, win.w[kk][a]);
+        for (int a = 0; a < 3; a++) direct.add(slab * kProfileSums + 4u + (uint32_t)a, wf[kk][a]);
       }
     }
   }
@@ -1921,10 +1926,10 @@ __device__ __forceinline__ void profile_flush(const ProfileSink& sink, uint32_t 
 }
 
 // WIDE: every replica's slabs are at least a cutoff wide (profile_part_wide,
-// no LDS bins); its main launch is held to 128 VGPRs, so that two 512-thread
-// workgroups share a CU.
+// no LDS bins); its main launch over dense bricks is held to 128 VGPRs, so
+// that two 512-thread workgroups share a CU.
 template <class L, bool SF, bool PARTS, bool WIDE>
-__global__ void __launch_bounds__(L::kThreads, WIDE && !PARTS ? 4 : 1) k_profile(Buffers B) {
+__global__ void __launch_bounds__(L::kThreads, WIDE && !PARTS && !L::kSparse ? 4 : 1) k_profile(Buffers B) {
   __shared__ Stage<L> S;
   // brick gb whole (code 0) or one part of it
   auto profile_wide = [&](uint32_t gb, uint32_t code) {
