@@ -259,12 +259,12 @@ int vlj_batch_read_particles(vlj_batch *batch, vlj_particle *out, size_t count);
  * least force time plus list-build time per step. It blocks; *steps_taken
  * gets the steps it ran (they count as steps of the run). A handle of at
  * least 65,536 atoms tunes itself (6 steps per candidate) on the first
- * vlj_step or vlj_run that asks for at least 64 steps. Every 512 steps the
+ * vlj_step or vlj_run that asks for at least 64 steps. Every 256 steps the
  * control words are copied back without a wait; a replica whose bricks
- * were cut into parts at more than 2% of its bricks per list build is
- * planned again from the occupancy of its last list build (its density
- * rose since), with its lists rebuilt for the new bricks. VLJ_E_STATE when
- * vlj_debug_plan fixes the plan. */
+ * were cut into parts since the check before is planned again from the
+ * occupancy of its last list build (its density rose since), held to 10%
+ * less of the staging limit each time (down to 60%), with its lists rebuilt
+ * for the new bricks. VLJ_E_STATE when vlj_debug_plan fixes the plan. */
 typedef struct {
     double fill;     /* the candidate's fill */
     double force_us; /* mean pair-force time per step */
