@@ -995,6 +995,51 @@ static_assert(kListStride == 64, "entry pos of a list lies (pos / 8) KB + 2 (pos
 // dx, then slot (R4), and a sparse cell's 27 neighbours share chunks.
 // On a reorder the workgroup also copies its home atoms' state back from
 // the second set.
+// Sparse bricks (dilute layers): a lane per home atom, walking its own
+// candidates in list order, the nine runs (dz, dy), each the three staged
+// cells along x, one range of staged atoms. A dilute cell holds a few atoms,
+// so the 64 candidates of a chunk (below) would be mostly empty lanes and
+// every home atom would cost a pass over its chunks; here every lane tests
+// only its own candidates. The lists are the same, entry for entry.
+template <class L>
+__device__ __forceinline__ void build_lanes(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
+                                            bool staged, const double* X, const double* Y, const double* Z) {
+  const uint32_t cap = R.capacity;
+  const uint32_t H = S.hoff[g.gx * g.gy];
+  for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
+    const Home a = home_atom(g, S, h);
+    double xi, yi, zi;
+    staged_pos(R, S, staged, a.e, X, Y, Z, xi, yi, zi);
+    const uint32_t li = a.slot - R.atom_off;
+    char* lst = (char*)(B.nbr + list_base(R, li));
+    uint32_t found = 0u;
+    for (uint32_t r = 0; r < 9u; r++) {
+      const uint32_t sc = staged_cell(g, r / 3u, a.hy + r % 3u, a.hx);
+      const uint32_t s1 = S.off[sc + 3u];
+      for (uint32_t q = S.off[sc]; q < s1; q++) {
+        double xj, yj, zj;
+        staged_pos(R, S, staged, q, X, Y, Z, xj, yj, zj);
+        const double d2 = dist2(xj - xi, yj - yi, zj - zi);
+        if (d2 < R.list_r2 && q != a.e) {
+          // byte offset (pos / 8) KB + 2 (pos % 8) = 2 pos + 1008 (pos / 8)
+          if (found < cap)
+            *(uint16_t*)(lst + (size_t)(__umul24(found >> 3, 1008u) + 2u * found)) =
+                (uint16_t)list_entry_of(q, staged);
+          found++;
+        }
+      }
+    }
+    // past the capacity the list keeps its first `cap` entries and the
+    // overflow is reported (lennard_jones.wgsl:494-497); the last octet is
+    // filled up with pads
+    const uint32_t len = found < cap ? found : cap;
+    uint16_t* l16 = B.nbr + list_base(R, li);
+    for (uint32_t p = len; (p & 7u) != 0u; p++) l16[list_entry(p)] = (uint16_t)entry_of(L::kListPad);
+    B.nbr_count[a.slot] = len;
+    if (found > cap) atomicMax(&B.ctl[g.rep].overflow, found);
+  }
+}
+
 // The lists of the home atoms of brick (or part) g, staged in S.
 // With `relist` the positions are those of the last build (xb) and the
 // state is left as it is: the same lists again, for another brick plan.
@@ -1037,6 +1082,10 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
   }
   if (total > 0xffffu) {
     if (threadIdx.x == 0 && H > 0u) atomicMax(&B.ctl[g.rep].overflow, total);
+    return;
+  }
+  if constexpr (L::kSparse) {
+    build_lanes(B, R, S, g, staged, X, Y, Z);
     return;
   }
   // Work items: the atoms of each home cell cut into pieces of at most
