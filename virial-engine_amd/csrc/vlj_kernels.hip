@@ -604,6 +604,8 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Buffers B) {
 // added, so a displacement is one subtraction (R4, R5). Every rebuild puts
 // the atoms in cell order (R8), so staged cell c holds the slots
 // start[c] .. start[c] + (off[c + 1] - off[c]).
+constexpr uint32_t kCoarse = 16;  // staged atoms per entry of Stage::coarse
+
 template <class L>
 struct Stage {
   // Staged positions and, at L::kListPad, the sentinel. They open the
@@ -617,6 +619,8 @@ struct Stage {
   uint8_t image[kStageCells];     // (ix + 1) + 3 (iy + 1) + 9 (iz + 1), i in {-1, 0, 1}
   uint32_t ns;                    // staged cells
   uint32_t total;                 // staged atoms
+  // the staged cell of every kCoarse-th staged atom (up to L::kStageMax)
+  uint16_t coarse[L::kStageMax / kCoarse];
 };
 
 // Staged index of the cell at offset (dz, yy, xx) from the brick's corner
@@ -625,10 +629,19 @@ __device__ __forceinline__ uint32_t staged_cell(const Brick& g, uint32_t dz, uin
   return (dz * (g.gy + 2u) + yy) * (g.gx + 2u) + xx;
 }
 
-// Staged cell of staged atom e (binary search over the staged offsets).
+// Staged cell of staged atom e: the last cell whose first staged atom is at
+// most e, by binary search over the staged offsets between the cells of
+// the kCoarse-th atoms at or below e and after it (a dense brick's cells
+// hold more than kCoarse atoms, so the search is over one or two cells).
 template <class L>
 __device__ __forceinline__ uint32_t stage_cell_of(const Stage<L>& S, uint32_t e) {
   uint32_t lo = 0, hi = S.ns - 1u;
+  const uint32_t known = S.total < (uint32_t)L::kStageMax ? S.total : (uint32_t)L::kStageMax;
+  if (e < known) {
+    const uint32_t m = e / kCoarse;
+    lo = S.coarse[m];
+    if ((m + 1u) * kCoarse < known) hi = S.coarse[m + 1u];
+  }
   while (lo < hi) {
     const uint32_t mid = (lo + hi + 1u) >> 1;
     if (S.off[mid] <= e) lo = mid; else hi = mid - 1u;
@@ -712,6 +725,19 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
     S.off[ns] = before + incl;
     S.total = before + incl;
     S.ns = ns;
+  }
+  __syncthreads();
+  {
+    // the coarse index: cell c holds staged atoms off[c] .. off[c + 1] - 1
+    const uint32_t known = S.total < (uint32_t)L::kStageMax ? S.total : (uint32_t)L::kStageMax;
+#pragma unroll
+    for (uint32_t h = 0; h < kCPT; h++) {
+      const uint32_t c = kCPT * t + h;
+      if (c < ns) {
+        const uint32_t end = S.off[c + 1u] < known ? S.off[c + 1u] : known;
+        for (uint32_t m = (S.off[c] + kCoarse - 1u) / kCoarse; m * kCoarse < end; m++) S.coarse[m] = (uint16_t)c;
+      }
+    }
   }
   __syncthreads();
   if (t < 64u) {  // home atoms, home cell by home cell (hy outer, hx inner): a scan over wave 0
