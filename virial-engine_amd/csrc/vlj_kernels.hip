@@ -613,10 +613,10 @@ struct Stage {
   // are one ds_read2st64_b64 and z[e] one ds_read_b64, all three addressed
   // by 8 e with no further offset.
   double x[L::kStageMax + 64], y[L::kStageMax + 64], z[L::kStageMax + 64];
-  uint32_t start[kStageCells];
+  uint32_t base[kStageCells];     // slot of staged atom e of cell c: base[c] + e (mod 2^32)
   uint32_t off[kStageCells + 1];
   uint32_t hoff[kHomeCells + 1];  // home atoms numbered home cell by home cell
-  uint8_t image[kStageCells];     // (ix + 1) + 3 (iy + 1) + 9 (iz + 1), i in {-1, 0, 1}
+  uint8_t image[kStageCells];     // (ix + 1) | (iy + 1) << 2 | (iz + 1) << 4, i in {-1, 0, 1}
   uint32_t ns;                    // staged cells
   uint32_t total;                 // staged atoms
   // the staged cell of every kCoarse-th staged atom (up to L::kStageMax)
@@ -649,10 +649,12 @@ __device__ __forceinline__ uint32_t stage_cell_of(const Stage<L>& S, uint32_t e)
   return lo;
 }
 
+constexpr uint32_t kNoImage = 1u | 1u << 2 | 1u << 4;  // the image code of no shift
+
 __device__ __forceinline__ void image_shift(const Replica& R, uint32_t code, double& sx, double& sy, double& sz) {
-  sx = (double)((int)(code % 3u) - 1) * R.box[0];
-  sy = (double)((int)((code / 3u) % 3u) - 1) * R.box[1];
-  sz = (double)((int)(code / 9u) - 1) * R.box[2];
+  sx = (double)((int)(code & 3u) - 1) * R.box[0];
+  sy = (double)((int)((code >> 2) & 3u) - 1) * R.box[1];
+  sz = (double)((int)(code >> 4) - 1) * R.box[2];
 }
 
 // Wave priorities (s_setprio). While a brick is staged: with two workgroups
@@ -685,23 +687,24 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
   const uint32_t t = threadIdx.x;
   // thread t stages the kCellsPerThread consecutive cells from t * kCellsPerThread
   constexpr uint32_t kCPT = (uint32_t)L::kCellsPerThread;
-  uint32_t n_c[kCPT];
+  uint32_t n_c[kCPT], st_c[kCPT];
 #pragma unroll
   for (uint32_t h = 0; h < kCPT; h++) {
     const uint32_t c = kCPT * t + h;
     n_c[h] = 0u;
+    st_c[h] = 0u;
     if (c < ns) {
       const int xx = (int)(c % sx), yy = (int)((c / sx) % sy), dz = (int)(c / (sx * sy));
       const int ux = cx0 - 1 + xx, uy = cy0 - 1 + yy, uz = cz - 1 + dz;  // unwrapped
       const int cx = (ux + nx) % nx, cy = (uy + ny) % ny, ccz = (uz + nz) % nz;
       const uint32_t cell = R.cell_off + (uint32_t)((ccz * ny + cy) * nx + cx);
-      S.start[c] = B.cell_start[cell];
+      st_c[h] = B.cell_start[cell];
       n_c[h] = B.cell_n[cell];
       // image of the cell seen from the brick (R4)
       const int ix = ux < 0 ? -1 : (ux >= nx ? 1 : 0);
       const int iy = uy < 0 ? -1 : (uy >= ny ? 1 : 0);
       const int iz = uz < 0 ? -1 : (uz >= nz ? 1 : 0);
-      S.image[c] = (uint8_t)((ix + 1) + 3 * (iy + 1) + 9 * (iz + 1));
+      S.image[c] = (uint8_t)((ix + 1) | (iy + 1) << 2 | (iz + 1) << 4);
     }
   }
   // exclusive scan of the staged counts over the workgroup
@@ -718,7 +721,10 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
   uint32_t run = before + incl - n_t;
 #pragma unroll
   for (uint32_t h = 0; h < kCPT; h++) {
-    if (kCPT * t + h < ns) S.off[kCPT * t + h] = run;
+    if (kCPT * t + h < ns) {
+      S.off[kCPT * t + h] = run;
+      S.base[kCPT * t + h] = st_c[h] - run;
+    }
     run += n_c[h];
   }
   if (t == (uint32_t)L::kThreads - 1u) {
@@ -761,11 +767,11 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       const uint32_t e = t + (uint32_t)k * L::kThreads;
-      code[k] = 13u;
+      code[k] = kNoImage;
       px[k] = py[k] = pz[k] = 0.0;
       if (e < total) {
         const uint32_t c = stage_cell_of(S, e);
-        const uint32_t j = S.start[c] + (e - S.off[c]);
+        const uint32_t j = S.base[c] + e;
         code[k] = S.image[c];
         px[k] = X[j];
         py[k] = Y[j];
@@ -811,7 +817,7 @@ __device__ __forceinline__ Home home_atom(const Brick& g, const Stage<L>& S, uin
   const uint32_t hs = staged_cell(g, 1u, a.hy + 1u, a.hx + 1u);
   const uint32_t idx = h - S.hoff[hc];
   a.e = S.off[hs] + idx;
-  a.slot = S.start[hs] + idx;
+  a.slot = S.base[hs] + a.e;
   return a;
 }
 
@@ -827,7 +833,7 @@ __device__ __forceinline__ void staged_pos(const Replica& R, const Stage<L>& S, 
     z = S.z[e];
   } else {
     const uint32_t c = stage_cell_of(S, e);
-    const uint32_t j = S.start[c] + (e - S.off[c]);
+    const uint32_t j = S.base[c] + e;
     double shx, shy, shz;
     image_shift(R, S.image[c], shx, shy, shz);
     x = X[j] + shx;
@@ -1137,7 +1143,7 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
     const uint32_t hx = hc % g.gx, hy = hc / g.gx;
     const uint32_t hs = staged_cell(g, 1u, hy + 1u, hx + 1u);
     const uint32_t ebase = uniform(S.off[hs]) - h0;                 // staged index of home atom k: ebase + k
-    const uint32_t lbase = uniform(S.start[hs]) - R.atom_off - h0;  // its replica slot: lbase + k
+    const uint32_t lbase = uniform(S.base[hs]) + ebase - R.atom_off;  // its replica slot: lbase + k
     // The candidates in list order: the nine runs (dz, dy), each the three
     // staged cells hx .. hx + 2 of row hy + dy of layer dz, one after the
     // other. Run r (lane r) holds sequence numbers rend[r - 1] .. rend[r] - 1;
@@ -2171,7 +2177,7 @@ __device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, 
       near += r2 < kNearR2 ? 1u : 0u;
       if (R.drawn == 1u) {
         const uint32_t c = stage_cell_of(S, e);
-        const uint32_t j = S.start[c] + (e - S.off[c]);
+        const uint32_t j = S.base[c] + e;
         v2 += dist2(B.vx[j], B.vy[j], B.vz[j]);
       }
     });
