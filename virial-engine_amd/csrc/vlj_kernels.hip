@@ -1174,10 +1174,13 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
         q[c] = L::kListPad;
         cx[c] = cy[c] = cz[c] = kFar;
         if (a >= T) continue;  // chunks past the candidates stay unread (a dilute item fills one)
-#pragma unroll
-        for (int r = 8; r >= 0; r--)  // the first run that ends past s
-          if (rend[r] > a && (r == 0 || rend[r - 1] < a + 64u) && s < rend[r]) q[c] = s + rdelta[r];
+        // the run s falls in: the last one whose predecessor ends at or
+        // before s (a chain of selects, no branch)
         const bool valid = s < T;
+        uint32_t d = rdelta[0];
+#pragma unroll
+        for (int r = 1; r < 9; r++) d = s >= rend[r - 1] ? rdelta[r] : d;
+        if (valid) q[c] = s + d;
         if (staged) {
           const uint32_t qs = valid ? q[c] : L::kListPad;
           cx[c] = S.x[qs];
