@@ -688,6 +688,7 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
   // thread t stages the kCellsPerThread consecutive cells from t * kCellsPerThread
   constexpr uint32_t kCPT = (uint32_t)L::kCellsPerThread;
   uint32_t n_c[kCPT], st_c[kCPT];
+  bool shifted_cell = false;  // a cell of this thread is a periodic image
 #pragma unroll
   for (uint32_t h = 0; h < kCPT; h++) {
     const uint32_t c = kCPT * t + h;
@@ -705,19 +706,27 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
       const int iy = uy < 0 ? -1 : (uy >= ny ? 1 : 0);
       const int iz = uz < 0 ? -1 : (uz >= nz ? 1 : 0);
       S.image[c] = (uint8_t)((ix + 1) | (iy + 1) << 2 | (iz + 1) << 4);
+      shifted_cell = shifted_cell || ix != 0 || iy != 0 || iz != 0;
     }
   }
-  // exclusive scan of the staged counts over the workgroup
+  // exclusive scan of the staged counts over the workgroup; each wave's
+  // total carries in its top bit whether the wave staged an image cell
   __shared__ uint32_t wtot[L::kThreads / 64];
   uint32_t n_t = 0;
 #pragma unroll
   for (uint32_t h = 0; h < kCPT; h++) n_t += n_c[h];
   const int lane = t & 63, wave = t >> 6;
   const uint32_t incl = wave_incl_scan_u32(n_t, lane);
-  if (lane == 63) wtot[wave] = incl;
+  const bool wave_shifted = __builtin_amdgcn_ballot_w64(shifted_cell) != 0ull;
+  if (lane == 63) wtot[wave] = incl | (wave_shifted ? 0x80000000u : 0u);
   __syncthreads();
-  uint32_t before = 0;
-  for (int w = 0; w < wave; w++) before += wtot[w];
+  uint32_t before = 0, shifted_any = 0;
+#pragma unroll
+  for (int w = 0; w < L::kThreads / 64; w++) {
+    const uint32_t x = wtot[w];
+    before += w < wave ? x & 0x7fffffffu : 0u;
+    shifted_any |= x >> 31;
+  }
   uint32_t run = before + incl - n_t;
 #pragma unroll
   for (uint32_t h = 0; h < kCPT; h++) {
@@ -778,15 +787,27 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
         pz[k] = Z[j];
       }
     }
+    if (shifted_any) {
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const uint32_t e = t + (uint32_t)k * L::kThreads;
-      if (e < total) {
-        double shx, shy, shz;
-        image_shift(R, code[k], shx, shy, shz);
-        S.x[e] = px[k] + shx;
-        S.y[e] = py[k] + shy;
-        S.z[e] = pz[k] + shz;
+      for (int k = 0; k < kPer; k++) {
+        const uint32_t e = t + (uint32_t)k * L::kThreads;
+        if (e < total) {
+          double shx, shy, shz;
+          image_shift(R, code[k], shx, shy, shz);
+          S.x[e] = px[k] + shx;
+          S.y[e] = py[k] + shy;
+          S.z[e] = pz[k] + shz;
+        }
+      }
+    } else {  // no cell of the brick is an image: every shift is +0
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        const uint32_t e = t + (uint32_t)k * L::kThreads;
+        if (e < total) {
+          S.x[e] = px[k] + 0.0;
+          S.y[e] = py[k] + 0.0;
+          S.z[e] = pz[k] + 0.0;
+        }
       }
     }
     if (t == 0) S.x[L::kListPad] = S.y[L::kListPad] = S.z[L::kListPad] = kFar;
