@@ -244,12 +244,13 @@ size_t vlj_batch_replicas(const vlj_batch *batch);
 int vlj_batch_read_particles(vlj_batch *batch, vlj_particle *out, size_t count);
 
 /* ---- brick planner and its tuner (no counterpart in the reference) ----
- * The kernels stage each brick's neighbourhood (gx x gy home cells of one z
- * layer and the cells around them) in LDS. Brick shapes are planned per
- * replica and z layer from the cell occupancy, the largest whose every
- * brick stages at most `fill` of the LDS staging limit (a window's count
- * plus four of its Poisson standard deviations); the layout (workgroup
- * size) follows from the staged count. Plans never change
+ * The kernels stage each brick's neighbourhood (gx x gy x gz home cells of
+ * one or two z layers and the cells around them) in LDS. Brick shapes are
+ * planned per replica and z layer from the cell occupancy, the largest whose
+ * every brick stages at most `fill` of the LDS staging limit (a window's
+ * count plus four of its Poisson standard deviations); two layers share
+ * bricks when that stages fewer cells per home cell (never with profiles).
+ * The layout (workgroup size) follows from the staged count. Plans never change
  * results: lists are built in the same order whatever the bricks.
  *
  * vlj_tune measures: for each candidate fill (0.8, 0.9, 1.0) it plans

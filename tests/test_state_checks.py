@@ -23,9 +23,12 @@ def sections(buf):
     rep_off = off
     nz = [struct.unpack_from("<3I", buf, rep_off + r * sz_rep + REPLICA_CELLS)[2] for r in range(R)]
     off += R * sz_rep
+    depth = []  # per replica: prof_samples, then brick_x, brick_y, brick_z and sparse per z layer
     for r in range(R):
-        off += 8 + nz[r] * 9
-    s = {"R": R, "N": N, "C": C_, "nb": nb, "rep": rep_off, "sz_rep": sz_rep, "bricks": off}
+        depth.append(off + 8 + nz[r] * 8)
+        off += 8 + nz[r] * 10
+    s = {"R": R, "N": N, "C": C_, "nb": nb, "rep": rep_off, "sz_rep": sz_rep, "depth": depth,
+         "sz_brick": sz_brick, "bricks": off}
     off += nb * sz_brick
     s["ctl"] = off
     off += R * sz_ctl
@@ -79,7 +82,8 @@ def u32(buf, at):
 
 
 @pytest.mark.parametrize("case", [
-    "brick_past_row", "bricks_overlap", "part_count", "bad_part", "cut_groups", "list_too_long",
+    "brick_past_row", "bricks_overlap", "brick_depth", "layer_depth", "part_count", "bad_part", "cut_groups",
+    "list_too_long",
     "cell_start", "cell_n", "cell_of", "orig_duplicate", "list_entry", "control_flag",
 ])
 def test_corrupted_fields_are_rejected(lj, state, tmp_path, case):
@@ -87,12 +91,15 @@ def test_corrupted_fields_are_rejected(lj, state, tmp_path, case):
 
     s = sections(state)
     nb, N = s["nb"], s["N"]
-    b0 = s["bricks"]  # Brick: rep, cell0, gx, gy
+    b0, b1 = s["bricks"], s["bricks"] + s["sz_brick"]  # Brick: rep, cell0, gx, gy, gz
     edits = {
         "brick_past_row": [(b0 + 8, "<I", 8)],
-        "bricks_overlap": [(b0 + 16 + 4, "<I", u32(state, b0 + 4)), (b0 + 16, "<I", u32(state, b0))],
+        "bricks_overlap": [(b1 + 4, "<I", u32(state, b0 + 4)), (b1, "<I", u32(state, b0))],
+        "brick_depth": [(b0 + 16, "<I", 3 - u32(state, b0 + 16))],
+        "layer_depth": [(s["depth"][0], "<B", 3)],
         "part_count": [(s["cut"], "<I", 65)],
-        "bad_part": [(s["cut"], "<I", 1), (s["parts"], "<H", 0 | (0 << 4) | (9 << 8) | (1 << 12)),
+        # part code: x0 | y0 << 3 | (w - 1) << 6 | (h - 1) << 9 | z0 << 12; x0 7 and w 8 pass any brick's edge
+        "bad_part": [(s["cut"], "<I", 1), (s["parts"], "<H", 7 | (0 << 3) | (7 << 6) | (0 << 9) | (0 << 12)),
                      (s["cut_groups"], "<I", 1)],
         "cut_groups": [(s["cut_groups"], "<I", 3)],
         "list_too_long": [(s["nbr_count"], "<I", 0xFFFF)],

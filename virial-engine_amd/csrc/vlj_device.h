@@ -139,9 +139,12 @@ struct Range {
 // A brick: gx x gy home cells of one z layer of a replica's grid, from the
 // cell (x0, y0, z) = cell0 (index within the replica). Its workgroup stages
 // the 3 x (gy + 2) x (gx + 2) cells around it.
+// gx x gy x gz home cells from cell0 (its lowest x, y and z) of replica
+// rep; gz is 1 or 2 cell layers.
 struct Brick {
-  uint32_t rep, cell0, gx, gy;
+  uint32_t rep, cell0, gx, gy, gz;
 };
+__host__ __device__ __forceinline__ uint32_t home_cells(const Brick& g) { return g.gx * g.gy * g.gz; }
 
 // Everything a kernel touches, passed by value.
 struct Buffers {

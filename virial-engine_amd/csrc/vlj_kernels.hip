@@ -624,7 +624,7 @@ struct Stage {
 };
 
 // Staged index of the cell at offset (dz, yy, xx) from the brick's corner
-// (-1, -1, -1): dz in 0..2, yy in 0..gy+1, xx in 0..gx+1.
+// (-1, -1, -1): dz in 0..gz+1, yy in 0..gy+1, xx in 0..gx+1.
 __device__ __forceinline__ uint32_t staged_cell(const Brick& g, uint32_t dz, uint32_t yy, uint32_t xx) {
   return (dz * (g.gy + 2u) + yy) * (g.gx + 2u) + xx;
 }
@@ -683,7 +683,7 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
   const int c0 = (int)g.cell0;
   const int cx0 = c0 % nx, cy0 = (c0 / nx) % ny, cz = c0 / (nx * ny);
   const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
-  const uint32_t ns = 3u * sy * sx;  // at most kStageCells
+  const uint32_t ns = (g.gz + 2u) * sy * sx;  // at most kStageCells
   const uint32_t t = threadIdx.x;
   // thread t stages the kCellsPerThread consecutive cells from t * kCellsPerThread
   constexpr uint32_t kCPT = (uint32_t)L::kCellsPerThread;
@@ -755,11 +755,11 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
     }
   }
   __syncthreads();
-  if (t < 64u) {  // home atoms, home cell by home cell (hy outer, hx inner): a scan over wave 0
-    const uint32_t nh = g.gx * g.gy;  // at most kHomeCells = 64
+  if (t < 64u) {  // home atoms, home cell by home cell (hz, hy outer, hx inner): a scan over wave 0
+    const uint32_t nh = home_cells(g);  // at most kHomeCells = 64
     uint32_t n = 0;
     if (t < nh) {
-      const uint32_t hs = staged_cell(g, 1u, t / g.gx + 1u, t % g.gx + 1u);
+      const uint32_t hs = staged_cell(g, t / (g.gx * g.gy) + 1u, (t / g.gx) % g.gy + 1u, t % g.gx + 1u);
       n = S.off[hs + 1u] - S.off[hs];
     }
     const uint32_t incl = wave_incl_scan_u32(n, lane);
@@ -821,21 +821,22 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
 
 // Home atom h of the brick: its home cell, staged index and slot.
 struct Home {
-  uint32_t hx, hy, e, slot;
+  uint32_t hx, hy, hz, e, slot;
 };
 
 template <class L>
 __device__ __forceinline__ Home home_atom(const Brick& g, const Stage<L>& S, uint32_t h) {
   // the last home cell whose first home atom is at most h (binary search)
-  uint32_t hc = 0, hi = g.gx * g.gy - 1u;
+  uint32_t hc = 0, hi = home_cells(g) - 1u;
   while (hc < hi) {
     const uint32_t mid = (hc + hi + 1u) >> 1;
     if (S.hoff[mid] <= h) hc = mid; else hi = mid - 1u;
   }
   Home a;
   a.hx = hc % g.gx;
-  a.hy = hc / g.gx;
-  const uint32_t hs = staged_cell(g, 1u, a.hy + 1u, a.hx + 1u);
+  a.hy = (hc / g.gx) % g.gy;
+  a.hz = hc / (g.gx * g.gy);
+  const uint32_t hs = staged_cell(g, a.hz + 1u, a.hy + 1u, a.hx + 1u);
   const uint32_t idx = h - S.hoff[hc];
   a.e = S.off[hs] + idx;
   a.slot = S.base[hs] + a.e;
@@ -867,28 +868,32 @@ __device__ __forceinline__ void staged_pos(const Replica& R, const Stage<L>& S, 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
 // A brick whose neighbourhood holds more than kStageMax atoms (the density
-// rose since the bricks were planned) is cut into parts: runs of whole home
-// rows, each run as long as still stages at most kStageMax atoms, and a home
-// row that does not fit alone cut the same way into runs of home cells. A
-// part of a single home cell that still does not fit reads its positions
-// from global memory. Every kernel that reads the lists cuts a brick the
-// same way, from the same cell counts, so the staged indices of a list refer
-// to the staging of its part. A part is packed as x0 | y0 << 4 | w << 8 |
-// h << 12 (home-cell offsets and sizes within the brick).
+// rose since the bricks were planned) is cut into parts: each of its cell
+// layers on its own, each layer into runs of whole home rows, each run as
+// long as still stages at most kStageMax atoms, and a home row that does
+// not fit alone cut the same way into runs of home cells. A part of a
+// single home cell that still does not fit reads its positions from global
+// memory. Every kernel that reads the lists cuts a brick the same way, from
+// the same cell counts, so the staged indices of a list refer to the
+// staging of its part. A part is packed as x0 | y0 << 3 | (w - 1) << 6 |
+// (h - 1) << 9 | z0 << 12 (home-cell offsets and sizes within the brick;
+// one layer).
 template <class L>
-__device__ __forceinline__ uint32_t part_rows(const Brick& g, const Stage<L>& S, uint32_t y0, uint32_t h) {
+__device__ __forceinline__ uint32_t part_rows(const Brick& g, const Stage<L>& S, uint32_t z0, uint32_t y0,
+                                              uint32_t h) {
   const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
   uint32_t n = 0;
-  for (uint32_t dz = 0; dz < 3u; dz++) n += S.off[(dz * sy + y0 + h + 2u) * sx] - S.off[(dz * sy + y0) * sx];
+  for (uint32_t dz = z0; dz < z0 + 3u; dz++)
+    n += S.off[(dz * sy + y0 + h + 2u) * sx] - S.off[(dz * sy + y0) * sx];
   return n;
 }
 
 template <class L>
-__device__ __forceinline__ uint32_t part_cells(const Brick& g, const Stage<L>& S, uint32_t y0, uint32_t x0,
-                                               uint32_t w) {
+__device__ __forceinline__ uint32_t part_cells(const Brick& g, const Stage<L>& S, uint32_t z0, uint32_t y0,
+                                               uint32_t x0, uint32_t w) {
   const uint32_t sx = g.gx + 2u, sy = g.gy + 2u;
   uint32_t n = 0;
-  for (uint32_t dz = 0; dz < 3u; dz++)
+  for (uint32_t dz = z0; dz < z0 + 3u; dz++)
     for (uint32_t yy = y0; yy < y0 + 3u; yy++) {
       const uint32_t row = (dz * sy + yy) * sx;
       n += S.off[row + x0 + w + 2u] - S.off[row + x0];
@@ -896,31 +901,38 @@ __device__ __forceinline__ uint32_t part_cells(const Brick& g, const Stage<L>& S
   return n;
 }
 
+__device__ __forceinline__ uint16_t part_code(uint32_t z0, uint32_t y0, uint32_t x0, uint32_t w, uint32_t h) {
+  return (uint16_t)(x0 | y0 << 3 | (w - 1u) << 6 | (h - 1u) << 9 | z0 << 12);
+}
+
 // The parts of brick g from its staged cell offsets (one thread).
 template <class L>
 __device__ uint32_t cut_brick(const Brick& g, const Stage<L>& S, uint16_t* parts) {
   const uint32_t lim = (uint32_t)L::kStageMax;
   uint32_t np = 0;
-  for (uint32_t y0 = 0; y0 < g.gy;) {
-    uint32_t h = 1;
-    while (y0 + h < g.gy && part_rows(g, S, y0, h + 1u) <= lim) h++;
-    if (h > 1u || g.gx == 1u || part_rows(g, S, y0, 1u) <= lim) {
-      parts[np++] = (uint16_t)(0u | (y0 << 4) | (g.gx << 8) | (h << 12));
-    } else {
-      for (uint32_t x0 = 0; x0 < g.gx;) {
-        uint32_t w = 1;
-        while (x0 + w < g.gx && part_cells(g, S, y0, x0, w + 1u) <= lim) w++;
-        parts[np++] = (uint16_t)(x0 | (y0 << 4) | (w << 8) | (1u << 12));
-        x0 += w;
+  for (uint32_t z0 = 0; z0 < g.gz; z0++)
+    for (uint32_t y0 = 0; y0 < g.gy;) {
+      uint32_t h = 1;
+      while (y0 + h < g.gy && part_rows(g, S, z0, y0, h + 1u) <= lim) h++;
+      if (h > 1u || g.gx == 1u || part_rows(g, S, z0, y0, 1u) <= lim) {
+        parts[np++] = part_code(z0, y0, 0u, g.gx, h);
+      } else {
+        for (uint32_t x0 = 0; x0 < g.gx;) {
+          uint32_t w = 1;
+          while (x0 + w < g.gx && part_cells(g, S, z0, y0, x0, w + 1u) <= lim) w++;
+          parts[np++] = part_code(z0, y0, x0, w, 1u);
+          x0 += w;
+        }
       }
+      y0 += h;
     }
-    y0 += h;
-  }
   return np;
 }
 
 __device__ __forceinline__ Brick part_brick(const Brick& g, uint32_t c, const Replica& R) {
-  return Brick{g.rep, g.cell0 + ((c >> 4) & 15u) * R.cells[0] + (c & 15u), (c >> 8) & 15u, c >> 12};
+  const uint32_t x0 = c & 7u, y0 = (c >> 3) & 7u, z0 = (c >> 12) & 1u;
+  return Brick{g.rep, g.cell0 + (z0 * R.cells[1] + y0) * R.cells[0] + x0, ((c >> 6) & 7u) + 1u, ((c >> 9) & 7u) + 1u,
+               1u};
 }
 
 // Workgroup-wide exclusive scan of one value per thread (THREADS threads);
@@ -1058,7 +1070,7 @@ template <class L>
 __device__ __forceinline__ void build_lanes(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                             bool staged, const double* X, const double* Y, const double* Z) {
   const uint32_t cap = R.capacity;
-  const uint32_t H = S.hoff[g.gx * g.gy];
+  const uint32_t H = S.hoff[home_cells(g)];
   for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home a = home_atom(g, S, h);
     double xi, yi, zi;
@@ -1067,7 +1079,7 @@ __device__ __forceinline__ void build_lanes(const Buffers& B, const Replica& R, 
     char* lst = (char*)(B.nbr + list_base(R, li));
     uint32_t found = 0u;
     for (uint32_t r = 0; r < 9u; r++) {
-      const uint32_t sc = staged_cell(g, r / 3u, a.hy + r % 3u, a.hx);
+      const uint32_t sc = staged_cell(g, a.hz + r / 3u, a.hy + r % 3u, a.hx);
       const uint32_t s1 = S.off[sc + 3u];
       for (uint32_t q = S.off[sc]; q < s1; q++) {
         double xj, yj, zj;
@@ -1103,7 +1115,7 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
   const uint32_t cap = R.capacity;
   const uint32_t total = S.total;
   if (!staged && !relist && threadIdx.x == 0) atomicAdd(&B.ctl[g.rep].stage_global, 1u);
-  const uint32_t nh = g.gx * g.gy;
+  const uint32_t nh = home_cells(g);
   const uint32_t H = S.hoff[nh];
   for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home a = home_atom(g, S, h);
@@ -1161,17 +1173,17 @@ __device__ __forceinline__ void build_part(const Buffers& B, const Replica& R, c
     const uint32_t h0 = readlane_u32(hv, hc), n = readlane_u32(hn, hc);
     const uint32_t p = item - readlane_u32(pv, hc), np = readlane_u32(pieces, hc);
     const uint32_t k0 = h0 + p * n / np, k1 = h0 + (p + 1u) * n / np;  // home atoms k0 .. k1 - 1
-    const uint32_t hx = hc % g.gx, hy = hc / g.gx;
-    const uint32_t hs = staged_cell(g, 1u, hy + 1u, hx + 1u);
+    const uint32_t hx = hc % g.gx, hy = (hc / g.gx) % g.gy, hz = hc / (g.gx * g.gy);
+    const uint32_t hs = staged_cell(g, hz + 1u, hy + 1u, hx + 1u);
     const uint32_t ebase = uniform(S.off[hs]) - h0;                 // staged index of home atom k: ebase + k
     const uint32_t lbase = uniform(S.base[hs]) + ebase - R.atom_off;  // its replica slot: lbase + k
     // The candidates in list order: the nine runs (dz, dy), each the three
-    // staged cells hx .. hx + 2 of row hy + dy of layer dz, one after the
-    // other. Run r (lane r) holds sequence numbers rend[r - 1] .. rend[r] - 1;
+    // staged cells hx .. hx + 2 of row hy + dy of layer hz + dz, one after
+    // the other. Run r (lane r) holds sequence numbers rend[r - 1] .. rend[r] - 1;
     // sequence number s of run r is staged atom s + rdelta[r].
     uint32_t rs = 0u, rl = 0u;
     if (lane < 9u) {
-      const uint32_t sc = staged_cell(g, lane / 3u, hy + lane % 3u, hx);
+      const uint32_t sc = staged_cell(g, hz + lane / 3u, hy + lane % 3u, hx);
       rs = S.off[sc];
       rl = S.off[sc + 3u] - rs;
     }
@@ -1285,7 +1297,7 @@ __global__ void __launch_bounds__(L::kThreads) k_build(Buffers B, int relist) {
     const double* Z = relist ? B.zb : (moved ? B.z2 : B.z);
     if (threadIdx.x == 0) next_item = 0u;
     const bool staged = stage_brick(B, R, g, X, Y, Z, S);  // ends with a barrier
-    const bool cut = !staged && g.gx * g.gy > 1u;
+    const bool cut = !staged && home_cells(g) > 1u;
     if (threadIdx.x == 0) {  // mark the brick and record its parts for the PARTS launches
       const uint32_t was = B.cut[gb];
       const uint32_t np = cut ? cut_brick(g, S, B.parts + (size_t)gb * kHomeCells) : 0u;
@@ -1578,7 +1590,7 @@ __device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica&
 template <int PREC, bool SF, class L>
 __device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                            bool staged, int kick) {
-  const uint32_t H = S.hoff[g.gx * g.gy];
+  const uint32_t H = S.hoff[home_cells(g)];
   for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
@@ -1608,7 +1620,7 @@ __global__ void __launch_bounds__(L::kThreads) k_force(Buffers B, int kick) {
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-    if (!staged && g.gx * g.gy > 1u) return;
+    if (!staged && home_cells(g) > 1u) return;
     force_part<PREC, SF>(B, R, S, g, staged, kick);
   } else {
     each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
@@ -1654,7 +1666,7 @@ __device__ __forceinline__ void block_add_slots(long long v[kSlots], long long* 
 template <int PREC, bool SF, class L>
 __device__ __forceinline__ void measure_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                              bool staged, long long* dst) {
-  const uint32_t H = S.hoff[g.gx * g.gy];
+  const uint32_t H = S.hoff[home_cells(g)];
   for (uint32_t h0 = 0; h0 < H; h0 += L::kThreads) {
     long long v[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
     const uint32_t h = h0 + threadIdx.x;
@@ -1689,7 +1701,7 @@ __global__ void __launch_bounds__(L::kThreads) k_measure(Buffers B) {
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-    if (!staged && g.gx * g.gy > 1u) return;
+    if (!staged && home_cells(g) > 1u) return;
     measure_part<PREC, SF>(B, R, S, g, staged, B.slots + (size_t)g.rep * kSlots);
   } else {
     each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
@@ -1781,7 +1793,7 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
   const double width = R.bin_width;
   const double iw = 1.0 / width;  // slab of z: floor(z iw)
   const int ibins = (int)bins;
-  const uint32_t H = S.hoff[g.gx * g.gy];
+  const uint32_t H = S.hoff[home_cells(g)];
   for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
@@ -1884,7 +1896,7 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
   const double iw = 1.0 / width;  // slab of z: floor(z iw)
   const int ibins = (int)bins;
   const ProfileSink direct{nullptr, prof, false};
-  const uint32_t H = S.hoff[g.gx * g.gy];
+  const uint32_t H = S.hoff[home_cells(g)];
   const uint32_t lane = threadIdx.x & 63u;
   // one wave-wide sum per slot of sum q, the lanes' values from `value`
   auto send = [&](int hmin, int q, auto value) {
@@ -2044,7 +2056,7 @@ __global__ void __launch_bounds__(L::kThreads, WIDE && !PARTS && !L::kSparse ? 4
     const Brick g = PARTS ? part_brick(brick, code, R) : brick;
     if (PARTS) __syncthreads();  // the previous part is done with S
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
-    if (!PARTS && !staged && g.gx * g.gy > 1u) return;
+    if (!PARTS && !staged && home_cells(g) > 1u) return;
     long long whole[3] = {0, 0, 0};
     long long* prof = B.profile + R.prof_off;
     profile_part_wide<SF>(B, R, S, g, staged, prof, whole);
@@ -2077,7 +2089,7 @@ __global__ void __launch_bounds__(L::kThreads, WIDE && !PARTS && !L::kSparse ? 4
     if (sink.use_lds)
       for (uint32_t k = threadIdx.x; k < nsums; k += L::kThreads) acc[k] = 0;
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
-    if (!PARTS && !staged && g.gx * g.gy > 1u) return;
+    if (!PARTS && !staged && home_cells(g) > 1u) return;
     long long whole[3] = {0, 0, 0};
     profile_part<SF>(B, R, S, g, staged, sink, whole);
     profile_flush<L>(sink, bins, whole);
@@ -2103,7 +2115,7 @@ __global__ void __launch_bounds__(L::kThreads, WIDE && !PARTS && !L::kSparse ? 4
 template <bool SF, class L>
 __device__ __forceinline__ void heat_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                           bool staged, long long jsum[kSlots]) {
-  const uint32_t H = S.hoff[g.gx * g.gy];
+  const uint32_t H = S.hoff[home_cells(g)];
   for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
@@ -2151,7 +2163,7 @@ __global__ void __launch_bounds__(L::kThreads) k_heat_current(Buffers B, uint32_
     const Replica R = B.rep[g.rep];
     if (R.hc_cap == 0u || sample >= R.hc_cap) return;
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-    if (!staged && g.gx * g.gy > 1u) return;
+    if (!staged && home_cells(g) > 1u) return;
     long long jsum[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
     heat_part<SF>(B, R, S, g, staged, jsum);
     block_add_slots<3, L::kThreads>(jsum, B.hc + 3ull * sample);
@@ -2180,7 +2192,7 @@ __global__ void __launch_bounds__(L::kThreads) k_heat_current(Buffers B, uint32_
 template <class L>
 __device__ __forceinline__ void render_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                             bool staged) {
-  const uint32_t H = S.hoff[g.gx * g.gy];
+  const uint32_t H = S.hoff[home_cells(g)];
   for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
@@ -2224,7 +2236,7 @@ __global__ void __launch_bounds__(L::kThreads) k_render(Buffers B) {
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-    if (!staged && g.gx * g.gy > 1u) return;
+    if (!staged && home_cells(g) > 1u) return;
     render_part(B, R, S, g, staged);
   } else {
     each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {

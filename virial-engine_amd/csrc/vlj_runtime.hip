@@ -62,6 +62,8 @@ struct HostReplica {
   vlj_settings settings;
   double energy_scale, momentum_scale;
   std::vector<uint32_t> brick_x, brick_y;  // brick shape in cells, per z layer
+  std::vector<uint8_t> brick_z;            // per z layer: home layers of the bricks from it (0: the layer
+                                           // below's bricks hold it)
   std::vector<uint8_t> sparse;             // per z layer: its bricks use the sparse layout
   double headroom = 1.0;  // share of the plan's fill this replica's bricks are held to
   uint64_t prof_samples;      // profile samples since the last clear
@@ -416,7 +418,8 @@ int capture(vlj_sim* s, uint32_t steps, hipGraphExec_t* out) {
 // the device. A layer whose bricks are held to at most `sparse_fill` of the
 // sparse layout's 1024 atoms takes that layout.
 void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fill, double sparse_fill,
-                   std::vector<uint32_t>& bx, std::vector<uint32_t>& by, std::vector<uint8_t>& sparse) {
+                   std::vector<uint32_t>& bx, std::vector<uint32_t>& by, std::vector<uint8_t>& bz,
+                   std::vector<uint8_t>& sparse) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
 #ifndef VLJ_MEAN_FLOOR
 #define VLJ_MEAN_FLOOR 1.0
@@ -426,18 +429,21 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fi
   // Two floors for a brick's count: the replica's mean occupancy (a
   // lattice filled from the bottom, with empty layers above, melts into
   // them; a gas condenses into droplets anywhere) sets the brick shape; the
-  // mean of the layer and its neighbours (a slab's vapour stays a vapour)
-  // sets the layout.
+  // mean of the layers it stages (a slab's vapour stays a vapour) sets the
+  // layout.
   const double whole = VLJ_MEAN_FLOOR * (double)R.count / (double)R.total_cells;
-  std::vector<double> layer_mean(nz, 0.0);
-  {
-    std::vector<uint64_t> layer(nz, 0);
-    for (int z = 0; z < nz; z++)
-      for (size_t c = 0; c < (size_t)nx * ny; c++) layer[z] += occ[(size_t)z * nx * ny + c];
-    for (int z = 0; z < nz; z++)
-      layer_mean[z] = VLJ_MEAN_FLOOR *
-                      (double)(layer[(z + nz - 1) % nz] + layer[z] + layer[(z + 1) % nz]) / (3.0 * nx * ny);
-  }
+  std::vector<uint64_t> layer(nz, 0);
+  for (int z = 0; z < nz; z++)
+    for (size_t c = 0; c < (size_t)nx * ny; c++) layer[z] += occ[(size_t)z * nx * ny + c];
+  auto wrap_z = [nz](int z) { return ((z % nz) + nz) % nz; };
+  // mean occupancy of the layers z - 1 .. z + k (what a brick of k home
+  // layers from z stages)
+  auto layers_mean = [&](int z, int k) {
+    uint64_t n = 0;
+    for (int i = -1; i <= k; i++) n += layer[wrap_z(z + i)];
+    return VLJ_MEAN_FLOOR * (double)n / ((double)(k + 2) * nx * ny);
+  };
+  // per layer, prefix sums of its counts over the periodically extended plane
   const int ex = nx + 2, ey = ny + 2;  // plane extended by one cell each side
   const size_t plane = (size_t)(ex + 1) * (ey + 1);
   std::vector<uint64_t> pre(plane * nz);
@@ -448,58 +454,103 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fi
         uint64_t v = 0;
         if (x > 0 && y > 0) {
           const int cx = (x - 2 + nx) % nx, cy = (y - 2 + ny) % ny;  // extended x - 1 is cell x - 2
-          for (int dz = -1; dz <= 1; dz++) v += occ[(size_t)(((z + dz + nz) % nz) * ny + cy) * nx + cx];
-          v += P[(size_t)y * (ex + 1) + (x - 1)] + P[(size_t)(y - 1) * (ex + 1) + x] -
-               P[(size_t)(y - 1) * (ex + 1) + (x - 1)];
+          v = occ[(size_t)(z * ny + cy) * nx + cx] + P[(size_t)y * (ex + 1) + (x - 1)] +
+              P[(size_t)(y - 1) * (ex + 1) + x] - P[(size_t)(y - 1) * (ex + 1) + (x - 1)];
         }
         P[(size_t)y * (ex + 1) + x] = v;
       }
   }
-  // staged cells of the brick at (x0, y0): x0 - 1 .. x0 + w, y0 - 1 .. y0 + h,
-  // i.e. extended columns x0 .. x0 + w + 1
-  auto window = [&](int z, int x0, int y0, int w, int h) {
-    const uint64_t* P = pre.data() + plane * z;
+  // atoms in the cells x0 - 1 .. x0 + w, y0 - 1 .. y0 + h of layer z (the
+  // extended columns x0 .. x0 + w + 1)
+  auto window1 = [&](int z, int x0, int y0, int w, int h) {
+    const uint64_t* P = pre.data() + plane * wrap_z(z);
     const int xa = x0, xb = x0 + w + 2, ya = y0, yb = y0 + h + 2;
     return P[(size_t)yb * (ex + 1) + xb] - P[(size_t)ya * (ex + 1) + xb] - P[(size_t)yb * (ex + 1) + xa] +
            P[(size_t)ya * (ex + 1) + xa];
+  };
+  // what a brick of k home layers from layer z stages: layers z - 1 .. z + k
+  auto staged_count = [&](int z, int k, int x0, int y0, int w, int h) {
+    uint64_t n = 0;
+    for (int i = -1; i <= k; i++) n += window1(z + i, x0, y0, w, h);
+    return n;
   };
   // A count held to the limit is the count plus four of its Poisson
   // standard deviations: a liquid's windows fluctuate about what the plan
   // saw (a lattice at the start), and a brick that outgrows LDS is cut.
   auto margin = [](uint64_t n) { return (double)n + 4.0 * std::sqrt((double)n); };
-  bx.assign(nz, 1);
-  by.assign(nz, 1);
-  sparse.assign(nz, 0);
-  for (int z = 0; z < nz; z++) {
+  struct Shape {
+    uint32_t gx = 1, gy = 1;
+    double worst = 0.0, local = 0.0;  // held count; held count by the staged layers' mean
+    bool fits = false;
+  };
+  // The largest gx x gy of k home layers from layer z whose every brick
+  // stages at most the limit, there and one layer up or down (a density
+  // profile may drift by a cell, as a liquid-vapour interface does), and at
+  // the replica's mean occupancy.
+  auto best_shape = [&](int z, int k) {
+    Shape b;
     uint32_t best = 0;
-    double best_worst = limit, best_local = limit;
     for (int gy = 1; gy <= std::min(kBrickMax, ny); gy++)
       for (int gx = gy; gx <= std::min(kBrickMax, nx); gx++) {
-        const uint32_t home = (uint32_t)(gx * gy);
-        if (home <= best) continue;
-        const double staged_cells = 3.0 * (gx + 2) * (gy + 2);
+        const uint32_t home = (uint32_t)(gx * gy * k);
+        if (home <= best || home > (uint32_t)kHomeCells || (k + 2) * (gx + 2) * (gy + 2) > kStageCells) continue;
+        const double staged_cells = (double)(k + 2) * (gx + 2) * (gy + 2);
         double windows = 0.0;  // the largest window, with its margin
         const double floor_count = margin((uint64_t)(whole * staged_cells));
-        for (int dz = -1; dz <= 1 && windows <= limit; dz++) {
-          const int zz = (z + dz + nz) % nz;
+        for (int dz = -1; dz <= 1 && windows <= limit; dz++)
           for (int y0 = 0; y0 < ny && windows <= limit; y0 += gy)
             for (int x0 = 0; x0 < nx && windows <= limit; x0 += gx)
-              windows = std::max(windows, margin(window(zz, x0, y0, std::min(gx, nx - x0), std::min(gy, ny - y0))));
-        }
+              windows = std::max(
+                  windows, margin(staged_count(z + dz, k, x0, y0, std::min(gx, nx - x0), std::min(gy, ny - y0))));
         const double worst = std::max(windows, floor_count);
         if (worst <= limit) {
           best = home;
-          best_worst = worst;
-          best_local = std::max(windows, margin((uint64_t)(layer_mean[z] * staged_cells)));
-          bx[z] = (uint32_t)gx;
-          by[z] = (uint32_t)gy;
+          b.gx = (uint32_t)gx;
+          b.gy = (uint32_t)gy;
+          b.worst = worst;
+          b.local = std::max(windows, margin((uint64_t)(layers_mean(z, k) * staged_cells)));
+          b.fits = true;
         }
       }
-    // a layer whose bricks stage few atoms takes the sparse layout: few by
-    // its own counts, and no more than twice the sparse limit by the
-    // replica's mean
-    sparse[z] = best > 0 && best_local <= sparse_limit && best_worst <= 2.0 * sparse_limit ? 1 : 0;
+    return b;
+  };
+  auto staged_per_home = [](const Shape& b, int k) {
+    return (double)(k + 2) * (b.gx + 2) * (b.gy + 2) / ((double)b.gx * b.gy * k);
+  };
+  // a brick that stages few atoms takes the sparse layout: few by its own
+  // layers' counts, and no more than twice the sparse limit by the
+  // replica's mean
+  auto sparse_of = [&](const Shape& b) {
+    return b.fits && b.local <= sparse_limit && b.worst <= 2.0 * sparse_limit ? 1 : 0;
+  };
+  bx.assign(nz, 1);
+  by.assign(nz, 1);
+  bz.assign(nz, 1);
+  sparse.assign(nz, 0);
+  std::vector<Shape> one(nz);
+  for (int z = 0; z < nz; z++) {
+    one[z] = best_shape(z, 1);
+    bx[z] = one[z].gx;
+    by[z] = one[z].gy;
+    sparse[z] = (uint8_t)sparse_of(one[z]);
   }
+  // Layers z, z + 1 (z even) share two-layer bricks when these stage fewer
+  // cells per home cell than the two layers' own bricks: a brick stages one
+  // layer below and one above its home layers, so two home layers stage
+  // four where two one-layer bricks stage six. Not for a replica with
+  // profiles, whose kernel takes a brick's atoms to lie in one cell layer.
+  if (R.prof_bins == 0 && nz >= 4)
+    for (int z = 0; z + 1 < nz; z += 2) {
+      const Shape two = best_shape(z, 2);
+      if (!two.fits) continue;
+      const double apart = 0.5 * (staged_per_home(one[z], 1) + staged_per_home(one[z + 1], 1));
+      if (!(staged_per_home(two, 2) < apart)) continue;
+      bx[z] = bx[z + 1] = two.gx;
+      by[z] = by[z + 1] = two.gy;
+      bz[z] = 2;
+      bz[z + 1] = 0;
+      sparse[z] = sparse[z + 1] = (uint8_t)sparse_of(two);
+    }
 }
 
 // The fills a plan is made with: the handle's (tuned) fill unless
@@ -527,18 +578,18 @@ void build_table(vlj_sim* s) {
       const Replica& R = s->rep[r];
       const HostReplica& H = s->host[r];
       for (uint32_t z = 0; z < R.cells[2]; z++) {
-        if ((H.sparse[z] != 0) != (pass == 1)) continue;
+        if (H.brick_z[z] == 0 || (H.sparse[z] != 0) != (pass == 1)) continue;  // layer held by the one below
         const uint32_t gx = H.brick_x[z], gy = H.brick_y[z];
         for (uint32_t y0 = 0; y0 < R.cells[1]; y0 += gy)
           for (uint32_t x0 = 0; x0 < R.cells[0]; x0 += gx) {
             s->bricks.push_back({(uint32_t)r, (z * R.cells[1] + y0) * R.cells[0] + x0,
-                                 std::min(gx, R.cells[0] - x0), std::min(gy, R.cells[1] - y0)});
+                                 std::min(gx, R.cells[0] - x0), std::min(gy, R.cells[1] - y0), H.brick_z[z]});
             s->rep_bricks[r]++;
           }
       }
     }
     std::stable_sort(s->bricks.begin() + (std::ptrdiff_t)pass_start, s->bricks.end(),
-                     [](const Brick& a, const Brick& b) { return a.gx * a.gy > b.gx * b.gy; });
+                     [](const Brick& a, const Brick& b) { return home_cells(a) > home_cells(b); });
     if (pass == 0) s->n_dense = (uint32_t)s->bricks.size();
   }
 }
@@ -613,7 +664,7 @@ int replan(vlj_sim* s, const std::vector<char>& which, double fill, double* buil
     std::vector<uint32_t> occ(cn.begin() + R.cell_off, cn.begin() + R.cell_off + R.total_cells);
     HostReplica& H = s->host[r];
     choose_bricks(R, occ, plan_fill(s) * H.headroom, plan_sparse_fill(s) * H.headroom, H.brick_x, H.brick_y,
-                  H.sparse);
+                  H.brick_z, H.sparse);
   }
   if (g_plan.stage_fill != 0.0) s->fill = saved;
   build_table(s);
@@ -746,9 +797,9 @@ int check_bricks(vlj_sim* s) {
         HostReplica& H = s->host[r];
         H.headroom = std::max(0.6, 0.9 * H.headroom);
         std::vector<uint32_t> bx, by;
-        std::vector<uint8_t> sp;
-        choose_bricks(rp, occ, plan_fill(s) * H.headroom, plan_sparse_fill(s) * H.headroom, bx, by, sp);
-        which[r] = bx != H.brick_x || by != H.brick_y || sp != H.sparse;
+        std::vector<uint8_t> bz, sp;
+        choose_bricks(rp, occ, plan_fill(s) * H.headroom, plan_sparse_fill(s) * H.headroom, bx, by, bz, sp);
+        which[r] = bx != H.brick_x || by != H.brick_y || bz != H.brick_z || sp != H.sparse;
         any = any || which[r];
       }
       if (any) {
@@ -830,7 +881,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     R.atom_block_off = (uint32_t)s->atom_blocks.size();
     R.atom_blocks = (R.count + kBlock - 1) / kBlock;
     choose_bricks(R, occ, plan_fill(s), plan_sparse_fill(s), s->host[r].brick_x, s->host[r].brick_y,
-                  s->host[r].sparse);
+                  s->host[r].brick_z, s->host[r].sparse);
     for (uint32_t b = 0; b < R.count; b += kBlock)
       s->atom_blocks.push_back({(uint32_t)r, atom_off + b, atom_off + std::min(R.count, b + kBlock), 0});
     for (uint32_t b = 0; b < R.total_cells; b += R.scan_cells)
@@ -1578,22 +1629,24 @@ int vlj_debug_lists(vlj_sim* sim, uint32_t* counts, uint32_t* entries) {
     for (uint32_t i = 0; i < R.count; i++) {
       const int c = (int)(cell_of[i] - R.cell_off);
       const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-      const int bx = (int)sim->host[0].brick_x[cz], by = (int)sim->host[0].brick_y[cz];
+      const HostReplica& H = sim->host[0];
+      const int z0 = H.brick_z[cz] == 0 ? cz - 1 : cz, gz = H.brick_z[z0];  // the brick's lowest layer
+      const int bx = (int)H.brick_x[z0], by = (int)H.brick_y[z0];
       const int x0 = cx / bx * bx, y0 = cy / by * by;
       const int gx = std::min(bx, nx - x0), gy = std::min(by, ny - y0);
       std::vector<uint32_t> off, first;  // staged offset and first slot of each staged cell
       uint32_t run = 0;
-      for (int dz = 0; dz < 3; dz++)
+      for (int dz = 0; dz < gz + 2; dz++)
         for (int yy = 0; yy < gy + 2; yy++)
           for (int xx = 0; xx < gx + 2; xx++) {
-            const int ccx = (x0 - 1 + xx + nx) % nx, ccy = (y0 - 1 + yy + ny) % ny, ccz = (cz - 1 + dz + nz) % nz;
+            const int ccx = (x0 - 1 + xx + nx) % nx, ccy = (y0 - 1 + yy + ny) % ny, ccz = (z0 - 1 + dz + nz) % nz;
             const int cell = (ccz * ny + ccy) * nx + ccx;
             off.push_back(run);
             first.push_back(start[cell] - R.atom_off);
             run += ncell[cell];
           }
       off.push_back(run);
-      const bool staged = run <= (uint32_t)(sim->host[0].sparse[cz] ? Sparse::kStageMax : Dense::kStageMax);
+      const bool staged = run <= (uint32_t)(H.sparse[z0] ? Sparse::kStageMax : Dense::kStageMax);
       const uint64_t base = list_base(R, i) - R.list_off;
       for (uint32_t k = 0; k < R.capacity; k++) {
         uint32_t slot = 0;
@@ -2045,7 +2098,7 @@ int vlj_random_insertion(size_t count, const double box[3], double min_distance,
 namespace {
 
 constexpr char kStateMagic[8] = {'V', 'L', 'J', 'S', 'T', 'A', 'T', 'E'};
-constexpr uint32_t kStateVersion = 1;
+constexpr uint32_t kStateVersion = 2;
 
 struct StateHeader {
   char magic[8];
@@ -2121,18 +2174,22 @@ std::string check_state(const std::vector<Replica>& rep, const std::vector<Brick
   std::vector<uint32_t> owner(h.cells_total, 0xffffffffu);
   for (size_t b = 0; b < nb; b++) {
     const Brick& g = bricks[b];
-    if (g.rep >= R || g.gx < 1 || g.gy < 1 || g.gx > (uint32_t)kBrickMax || g.gy > (uint32_t)kBrickMax)
+    if (g.rep >= R || g.gx < 1 || g.gy < 1 || g.gz < 1 || g.gz > 2u || g.gx > (uint32_t)kBrickMax ||
+        g.gy > (uint32_t)kBrickMax || home_cells(g) > (uint32_t)kHomeCells ||
+        (g.gz + 2u) * (g.gx + 2u) * (g.gy + 2u) > (uint32_t)kStageCells)
       return "bad brick table";
     const Replica& x = rep[g.rep];
     if (g.cell0 >= x.total_cells) return "bad brick table";
     const uint32_t cx = g.cell0 % x.cells[0], cy = (g.cell0 / x.cells[0]) % x.cells[1];
-    if (cx + g.gx > x.cells[0] || cy + g.gy > x.cells[1]) return "a brick runs past its row";
-    for (uint32_t yy = 0; yy < g.gy; yy++)
-      for (uint32_t xx = 0; xx < g.gx; xx++) {
-        uint32_t& o = owner[x.cell_off + g.cell0 + yy * x.cells[0] + xx];
-        if (o != 0xffffffffu) return "bricks overlap";
-        o = (uint32_t)b;
-      }
+    const uint32_t cz = g.cell0 / (x.cells[0] * x.cells[1]);
+    if (cx + g.gx > x.cells[0] || cy + g.gy > x.cells[1] || cz + g.gz > x.cells[2]) return "a brick runs past its row";
+    for (uint32_t zz = 0; zz < g.gz; zz++)
+      for (uint32_t yy = 0; yy < g.gy; yy++)
+        for (uint32_t xx = 0; xx < g.gx; xx++) {
+          uint32_t& o = owner[x.cell_off + g.cell0 + (zz * x.cells[1] + yy) * x.cells[0] + xx];
+          if (o != 0xffffffffu) return "bricks overlap";
+          o = (uint32_t)b;
+        }
   }
   for (uint32_t o : owner)
     if (o == 0xffffffffu) return "bricks do not cover every cell";
@@ -2145,8 +2202,9 @@ std::string check_state(const std::vector<Replica>& rep, const std::vector<Brick
     if (np) group_cut[b / kCutGroup]++;
     for (uint32_t u = 0; u < np; u++) {
       const uint32_t c = a.parts[b * kHomeCells + u];
-      const uint32_t x0 = c & 15u, y0 = (c >> 4) & 15u, w = (c >> 8) & 15u, hh = c >> 12;
-      if (w < 1 || hh < 1 || x0 + w > bricks[b].gx || y0 + hh > bricks[b].gy) return "bad part of a cut brick";
+      const uint32_t x0 = c & 7u, y0 = (c >> 3) & 7u, w = ((c >> 6) & 7u) + 1u, hh = ((c >> 9) & 7u) + 1u,
+                     z0 = c >> 12;
+      if (x0 + w > bricks[b].gx || y0 + hh > bricks[b].gy || z0 >= bricks[b].gz) return "bad part of a cut brick";
     }
   }
   for (size_t g = 0; g < ng; g++)
@@ -2181,22 +2239,25 @@ std::string check_state(const std::vector<Replica>& rep, const std::vector<Brick
       Brick g = whole;
       if (np) {
         const uint32_t c = a.parts[b * kHomeCells + u];
-        g = Brick{whole.rep, whole.cell0 + ((c >> 4) & 15u) * x.cells[0] + (c & 15u), (c >> 8) & 15u, c >> 12};
+        const uint32_t x0 = c & 7u, y0 = (c >> 3) & 7u, z0 = (c >> 12) & 1u;
+        g = Brick{whole.rep, whole.cell0 + (z0 * x.cells[1] + y0) * x.cells[0] + x0, ((c >> 6) & 7u) + 1u,
+                  ((c >> 9) & 7u) + 1u, 1u};
       }
       const int nx = (int)x.cells[0], ny = (int)x.cells[1], nz = (int)x.cells[2];
       const int cx0 = (int)(g.cell0 % x.cells[0]), cy0 = (int)((g.cell0 / x.cells[0]) % x.cells[1]);
       const int cz = (int)(g.cell0 / (x.cells[0] * x.cells[1]));
       uint64_t total = 0;
-      for (int dz = 0; dz < 3; dz++)
+      for (int dz = 0; dz < (int)g.gz + 2; dz++)
         for (int yy = 0; yy < (int)g.gy + 2; yy++)
           for (int xx = 0; xx < (int)g.gx + 2; xx++) {
             const int ccx = (cx0 - 1 + xx + nx) % nx, ccy = (cy0 - 1 + yy + ny) % ny, ccz = (cz - 1 + dz + nz) % nz;
             total += a.cell_n[x.cell_off + (uint32_t)((ccz * ny + ccy) * nx + ccx)];
           }
       const bool staged = total <= stage_max;
+      for (uint32_t zz = 0; zz < g.gz; zz++)
       for (uint32_t yy = 0; yy < g.gy; yy++)
         for (uint32_t xx = 0; xx < g.gx; xx++) {
-          const uint32_t c = x.cell_off + g.cell0 + yy * x.cells[0] + xx;
+          const uint32_t c = x.cell_off + g.cell0 + (zz * x.cells[1] + yy) * x.cells[0] + xx;
           for (uint32_t i = a.cell_start[c]; i < a.cell_start[c] + a.cell_n[c]; i++) {
             const uint64_t base = list_base(x, i - x.atom_off);
             for (uint32_t k = 0; k < a.nbr_count[i]; k++) {
@@ -2255,6 +2316,7 @@ int vlj_save(vlj_sim* sim, const char* path) {
     w.put(&x.prof_samples, sizeof(uint64_t));
     w.put(x.brick_x.data(), x.brick_x.size() * sizeof(uint32_t));
     w.put(x.brick_y.data(), x.brick_y.size() * sizeof(uint32_t));
+    w.put(x.brick_z.data(), x.brick_z.size());
     w.put(x.sparse.data(), x.sparse.size());
   }
   w.put(sim->bricks.data(), nb * sizeof(Brick));
@@ -2369,12 +2431,17 @@ int vlj_load(const char* path, vlj_sim** out) {
   if (atoms != h.n_total || cells != h.cells_total) return bad("inconsistent sizes");
   std::vector<uint64_t> prof_samples(R);
   std::vector<std::vector<uint32_t>> bx(R), by(R);
-  std::vector<std::vector<uint8_t>> sp(R);
+  std::vector<std::vector<uint8_t>> bz(R), sp(R);
   for (size_t k = 0; k < R; k++) {
     r.get(&prof_samples[k], sizeof(uint64_t));
     if (!read_vec(r, bx[k], rep[k].cells[2]) || !read_vec(r, by[k], rep[k].cells[2]) ||
-        !read_vec(r, sp[k], rep[k].cells[2]))
+        !read_vec(r, bz[k], rep[k].cells[2]) || !read_vec(r, sp[k], rep[k].cells[2]))
       return bad("truncated");
+    // Depths: 1, or 2 followed by the 0 of the layer that brick also holds.
+    const std::vector<uint8_t>& d = bz[k];
+    for (size_t z = 0; z < d.size(); z++)
+      if (d[z] > 2u || (d[z] == 2u && (z + 1 == d.size() || d[z + 1] != 0u)) || (d[z] == 0u && (z == 0 || d[z - 1] != 2u)))
+        return bad("replica " + std::to_string(k) + ": bad brick depth of layer " + std::to_string(z));
   }
   if (h.n_dense > h.n_bricks) return bad("bad brick table");
   std::vector<Brick> bricks;
@@ -2392,6 +2459,15 @@ int vlj_load(const char* path, vlj_sim** out) {
   {
     std::string why = check_state(rep, bricks, h, ctl, a);
     if (!why.empty()) return fail(nullptr, VLJ_E_INVALID, std::string(path) + ": " + why);
+  }
+  for (const Brick& g : bricks) {  // each brick as its layer's shape gives it
+    const Replica& x = rep[g.rep];
+    const uint32_t nx = x.cells[0], ny = x.cells[1], z = g.cell0 / (nx * ny);
+    if (z >= bz[g.rep].size()) return fail(nullptr, VLJ_E_INVALID, std::string(path) + ": bad brick table");
+    const uint32_t y0 = (g.cell0 / nx) % ny, x0 = g.cell0 % nx, bw = bx[g.rep][z], bh = by[g.rep][z];
+    if (bw == 0 || bh == 0 || g.gz != bz[g.rep][z] || x0 % bw != 0 || y0 % bh != 0 || g.gx != std::min(bw, nx - x0) ||
+        g.gy != std::min(bh, ny - y0))
+      return fail(nullptr, VLJ_E_INVALID, std::string(path) + ": a brick does not match its layer's shape");
   }
   // The particles in original order, to build the handle from.
   std::vector<vlj_particle> parts(N);
@@ -2457,6 +2533,7 @@ int vlj_load(const char* path, vlj_sim** out) {
     s->host[k].prof_samples = prof_samples[k];
     s->host[k].brick_x = bx[k];
     s->host[k].brick_y = by[k];
+    s->host[k].brick_z = bz[k];
     s->host[k].sparse = sp[k];
   }
   s->any_heat_exchange = false;
