@@ -249,7 +249,7 @@ int vlj_batch_read_particles(vlj_batch *batch, vlj_particle *out, size_t count);
  * planned per replica and z layer from the cell occupancy, the largest whose
  * every brick stages at most `fill` of the LDS staging limit (a window's
  * count plus four of its Poisson standard deviations); two layers share
- * bricks when that stages fewer cells per home cell (never with profiles).
+ * bricks when that stages fewer cells per home cell.
  * The layout (workgroup size) follows from the staged count. Plans never change
  * results: lists are built in the same order whatever the bricks.
  *

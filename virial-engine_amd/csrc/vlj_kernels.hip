@@ -1872,8 +1872,8 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
 
 // Slabs at least a cutoff wide: a wave's atoms put all their items in the
 // slabs hmin - 1 .. hmin + kWaveSlots - 2, hmin the lowest home slab in the
-// wave (a brick covers one cell layer, so a wave's home slabs span one or
-// two). Each (slab, sum) of these is added over the wave's lanes first and
+// wave (a wave's home atoms lie in a few consecutive cells of one or two
+// cell layers, so its home slabs mostly span one or two). Each (slab, sum) of these is added over the wave's lanes first and
 // sent to the profile by one atomic, with no LDS bins (two workgroups per
 // CU instead of one). A wave whose home slabs spread wider sends each
 // lane's items on their own.

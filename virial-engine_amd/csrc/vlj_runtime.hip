@@ -421,9 +421,6 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fi
                    std::vector<uint32_t>& bx, std::vector<uint32_t>& by, std::vector<uint8_t>& bz,
                    std::vector<uint8_t>& sparse) {
   const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
-#ifndef VLJ_MEAN_FLOOR
-#define VLJ_MEAN_FLOOR 1.0
-#endif
   const double limit = fill * kStageMax;
   const double sparse_limit = sparse_fill * Sparse::kStageMax;
   // Two floors for a brick's count: the replica's mean occupancy (a
@@ -431,7 +428,7 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fi
   // them; a gas condenses into droplets anywhere) sets the brick shape; the
   // mean of the layers it stages (a slab's vapour stays a vapour) sets the
   // layout.
-  const double whole = VLJ_MEAN_FLOOR * (double)R.count / (double)R.total_cells;
+  const double whole = (double)R.count / (double)R.total_cells;
   std::vector<uint64_t> layer(nz, 0);
   for (int z = 0; z < nz; z++)
     for (size_t c = 0; c < (size_t)nx * ny; c++) layer[z] += occ[(size_t)z * nx * ny + c];
@@ -441,7 +438,7 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fi
   auto layers_mean = [&](int z, int k) {
     uint64_t n = 0;
     for (int i = -1; i <= k; i++) n += layer[wrap_z(z + i)];
-    return VLJ_MEAN_FLOOR * (double)n / ((double)(k + 2) * nx * ny);
+    return (double)n / ((double)(k + 2) * nx * ny);
   };
   // per layer, prefix sums of its counts over the periodically extended plane
   const int ex = nx + 2, ey = ny + 2;  // plane extended by one cell each side
@@ -537,9 +534,8 @@ void choose_bricks(const Replica& R, const std::vector<uint32_t>& occ, double fi
   // Layers z, z + 1 (z even) share two-layer bricks when these stage fewer
   // cells per home cell than the two layers' own bricks: a brick stages one
   // layer below and one above its home layers, so two home layers stage
-  // four where two one-layer bricks stage six. Not for a replica with
-  // profiles, whose kernel takes a brick's atoms to lie in one cell layer.
-  if (R.prof_bins == 0 && nz >= 4)
+  // four where two one-layer bricks stage six.
+  if (nz >= 4)
     for (int z = 0; z + 1 < nz; z += 2) {
       const Shape two = best_shape(z, 2);
       if (!two.fits) continue;
