@@ -37,9 +37,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "virial-engine_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
-# Algorithmic bytes of one force-kernel launch per atom: read x_i and v_i,
-# write a_i and v_i, 24 bytes each in FP64.
-FORCE_BYTES_PER_ATOM = 96.0
+# Algorithmic bytes of one force-kernel launch per atom in the NVE form,
+# where the launch also drifts the next step's positions: read x_i, v_i, a_i
+# and x_i at the list build; write a_i, v_i and the next x_i; 24 bytes each
+# in FP64.
+FORCE_BYTES_PER_ATOM = 168.0
 # Flops of one pair-force evaluation (displacement 3, r^2 5, 1/r^2 1, s^6 2,
 # s^12 1, r.F 3, F/r^2 1, accumulate 6) and MI355X's published FP64 vector peak.
 PAIR_FLOPS = 22.0
@@ -496,7 +498,7 @@ def main():
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic_bytes,
-        "kernel": FORCE_KERNEL + ": FP64 pair forces from brick-staged lists + second half kick",
+        "kernel": FORCE_KERNEL + ": FP64 pair forces from brick-staged lists, the velocity step and the next step's drift",
         "algorithmic_bytes_per_launch": FORCE_BYTES_PER_ATOM * n,
         "avg_launch_us": fp64["force_kernel_us"],
         "measured_copy_GBs": copy_gbs,

@@ -177,10 +177,14 @@ struct Buffers {
   long long* profile;                                        // profile sums of all replicas
   float *render_pos, *render_vel;                            // float4 per slot, for drawing
   uint32_t* tickets;                                         // last-block counters
-  // flags[0]: replicas rebuilding this step (zeroed by the drift, counted by
-  // the decision); flags[1]: bricks cut into parts. Kernels that have
-  // nothing to do without them return after one load.
+  // flags[1]: bricks cut into parts; flags[0] and flags[2]: replicas
+  // rebuilding on a step, one word for each of two consecutive steps of the
+  // drift-ahead form. Kernels that have nothing to do without them return
+  // after one load.
   uint32_t* flags;
+  // this step's word (zeroed by the drift, or by the decision of the step
+  // before, and counted by the decision) and the next step's
+  uint32_t *rebuilding, *rebuilding_next;
 };
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on

@@ -82,13 +82,16 @@ __device__ void decide_from(const Buffers& B, uint32_t r, double moved2, bool ad
   if (rebuild) {
     c->rebuilds = rebuilds + 1u;
     c->built = 1u;
-    atomicAdd(&B.flags[0], 1u);
+    atomicAdd(B.rebuilding, 1u);
   }
   if (advance_step) c->step = c->step + 1ull;
 }
 
 // One workgroup per replica: the largest moved^2 over the replica's drift
-// workgroups (bmax), then the decision. `advance` also counts the step.
+// workgroups (bmax), then the decision. `advance` also counts the step and
+// sets each bmax it read back to zero (the drift-ahead force takes the
+// maximum into them); the first workgroup zeroes the next step's rebuild
+// word, which no kernel of this step reads.
 __global__ void __launch_bounds__(kBlock) k_decide(Buffers B, int advance) {
   const uint32_t r = blockIdx.x;
   const Replica R = B.rep[r];
@@ -102,8 +105,12 @@ __global__ void __launch_bounds__(kBlock) k_decide(Buffers B, int advance) {
 #pragma unroll
       for (uint32_t j = 0; j < 16u; j++) v[j] = b0 + j * kBlock < end ? B.bmax[b0 + j * kBlock] : 0ull;
 #pragma unroll
-      for (uint32_t j = 0; j < 16u; j++) m = v[j] > m ? v[j] : m;
+      for (uint32_t j = 0; j < 16u; j++) {
+        m = v[j] > m ? v[j] : m;
+        if (b0 + j * kBlock < end) B.bmax[b0 + j * kBlock] = 0ull;
+      }
     }
+    if (r == 0 && threadIdx.x == 0) *B.rebuilding_next = 0u;
   }
   __shared__ unsigned long long wmax[kBlock / 64];
   m = wave_max_u64(m);
@@ -121,9 +128,11 @@ __global__ void __launch_bounds__(kBlock) k_decide(Buffers B, int advance) {
 // ---------------------------------------------------------------------------
 
 // LANGEVIN: some replica of the launch uses BAOAB (others stay NVE).
+// keep_v: the velocities stay v(t) (the drift-ahead force forms the half
+// step again, as here, before its kick); only the positions move.
 template <bool LANGEVIN>
-__global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) B.flags[0] = 0u;  // counted again by k_decide
+__global__ void __launch_bounds__(kBlock) k_drift(Buffers B, int keep_v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *B.rebuilding = 0u;  // counted again by k_decide
   const uint32_t b = dealt_index(blockIdx.x);
   if (b >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[b];
@@ -154,7 +163,7 @@ __global__ void __launch_bounds__(kBlock) k_drift(Buffers B) {
       } else {
         x = __builtin_fma(v, R.dt, x);
       }
-      V[k][i] = v;
+      if (!keep_v) V[k][i] = v;
       X[k][i] = x;  // wrapped at the next rebuild (R1)
       d[k] = x - XB[k][i];
     }
@@ -402,7 +411,7 @@ __device__ __forceinline__ uint32_t run_atomic_add(uint32_t* counter, uint32_t k
 }
 
 __global__ void __launch_bounds__(kBlock) k_bin(Buffers B) {
-  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
+  if (*B.rebuilding == 0u) return;  // no replica rebuilds this step
   const uint32_t ab = dealt_index(blockIdx.x);
   if (ab >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[ab];
@@ -426,7 +435,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(Buffers B) {
 // Exclusive scan of counts inside each block of 1024 cells (4 per thread);
 // the last workgroup scans the block totals of each rebuilding replica.
 __global__ void __launch_bounds__(kBlock) k_scan(Buffers B) {
-  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
+  if (*B.rebuilding == 0u) return;  // no replica rebuilds this step
   const Range blk = B.scan_blocks[blockIdx.x];
   const bool active = B.ctl[blk.rep].rebuild != 0u;
   __shared__ uint32_t wsum[kBlock / 64];
@@ -491,7 +500,7 @@ __device__ __forceinline__ uint32_t cell_first_slot(const Buffers& B, const Repl
 }
 
 __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
-  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
+  if (*B.rebuilding == 0u) return;  // no replica rebuilds this step
   const uint32_t ab = dealt_index(blockIdx.x);
   if (ab >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[ab];
@@ -513,7 +522,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Buffers B) {
 // one: there a thread ranks a cell of at most kSmallCell atoms on its own,
 // and the waves take the larger cells of the block in turn.
 __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
-  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
+  if (*B.rebuilding == 0u) return;  // no replica rebuilds this step
   const uint32_t sb = dealt_index(blockIdx.x);
   if (sb >= B.n_sort_blocks) return;
   const Range blk = B.sort_blocks[sb];
@@ -573,7 +582,7 @@ __global__ void __launch_bounds__(kBlock) k_sort(Buffers B) {
 // Slot t takes the atom at sorted[t] (lennard_jones.wgsl:419-452); the copy
 // goes to the second state set and `build` copies it back.
 __global__ void __launch_bounds__(kBlock) k_reorder(Buffers B) {
-  if (B.flags[0] == 0u) return;  // no replica rebuilds this step
+  if (*B.rebuilding == 0u) return;  // no replica rebuilds this step
   const uint32_t ab = dealt_index(blockIdx.x);
   if (ab >= B.n_atom_blocks) return;
   const Range blk = B.atom_blocks[ab];
@@ -1283,7 +1292,7 @@ template <class L, bool PARTS>
 __global__ void __launch_bounds__(L::kThreads) k_build(Buffers B, int relist) {
   __shared__ Stage<L> S;
   __shared__ uint32_t next_item;
-  if (!relist && B.flags[0] == 0u) return;  // no replica rebuilds this step
+  if (!relist && *B.rebuilding == 0u) return;  // no replica rebuilds this step
   if constexpr (!PARTS) {
     const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
     if (gb >= layout_end<L>(B)) return;
@@ -1585,27 +1594,58 @@ __device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica&
   return interactions_global<PREC, SF, ENERGIES>(B, R, S, i, xi, yi, zi);
 }
 
-// New accelerations and (kick) the second half kick v += a dt/2, one
-// workgroup per brick.
+// New accelerations, one workgroup per brick (gb: its table entry), and
+// with kick 1 the second half kick v += a dt/2 (v holds the half step).
+// Kick 2, the drift-ahead form (NVE): v holds v(t) and a a(t); the half
+// step v(t) + a(t) dt/2 is formed again exactly as the drift forms it, the
+// kick gives v(t + dt), and the next step's drift follows at once: its half
+// step and x(t + 2 dt) = x(t + dt) + v' dt go to the second position set
+// (x2), which no kernel of this step reads, and the largest moved^2 since
+// the list build into one of the replica's drift maxima (bmax, zeroed by
+// the decision that read them), so the next step needs no drift launch.
 template <int PREC, bool SF, class L>
 __device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
-                                           bool staged, int kick) {
+                                           uint32_t gb, bool staged, int kick) {
   const uint32_t H = S.hoff[home_cells(g)];
+  unsigned long long moved = 0ull;
   for (uint32_t h = threadIdx.x; h < H; h += L::kThreads) {
     const Home at = home_atom(g, S, h);
     const uint32_t i = at.slot;
     double xi, yi, zi;
     staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
     const PairAcc G = interactions<PREC, SF, false>(B, R, S, staged, i, xi, yi, zi);
-    const double ax = -24.0 * G.ax, ay = -24.0 * G.ay, az = -24.0 * G.az;
-    B.ax[i] = ax;
-    B.ay[i] = ay;
-    B.az[i] = az;
-    if (kick) {
-      B.vx[i] = __builtin_fma(ax, R.half_dt, B.vx[i]);
-      B.vy[i] = __builtin_fma(ay, R.half_dt, B.vy[i]);
-      B.vz[i] = __builtin_fma(az, R.half_dt, B.vz[i]);
+    const double a[3] = {-24.0 * G.ax, -24.0 * G.ay, -24.0 * G.az};
+    double* const A[3] = {B.ax, B.ay, B.az};
+    double* const V[3] = {B.vx, B.vy, B.vz};
+    if (kick == 2) {
+      const double* const X[3] = {B.x, B.y, B.z};
+      double* const X2[3] = {B.x2, B.y2, B.z2};
+      const double* const XB[3] = {B.xb, B.yb, B.zb};
+      double d[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double half = __builtin_fma(A[k][i], R.half_dt, V[k][i]);  // the drift's half step
+        const double v = __builtin_fma(a[k], R.half_dt, half);
+        A[k][i] = a[k];
+        V[k][i] = v;
+        const double x = __builtin_fma(__builtin_fma(a[k], R.half_dt, v), R.dt, X[k][i]);
+        X2[k][i] = x;
+        d[k] = x - XB[k][i];
+      }
+      const unsigned long long m = (unsigned long long)__double_as_longlong(dist2(d[0], d[1], d[2]));
+      moved = m > moved ? m : moved;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        A[k][i] = a[k];
+        if (kick) V[k][i] = __builtin_fma(a[k], R.half_dt, V[k][i]);
+      }
     }
+  }
+  if (kick == 2) {
+    moved = wave_max_u64(moved);
+    if ((threadIdx.x & 63u) == 0 && moved != 0ull && R.atom_blocks != 0u)
+      atomicMax(B.bmax + R.atom_block_off + gb % R.atom_blocks, moved);
   }
 }
 
@@ -1621,7 +1661,7 @@ __global__ void __launch_bounds__(L::kThreads) k_force(Buffers B, int kick) {
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
     if (!staged && home_cells(g) > 1u) return;
-    force_part<PREC, SF>(B, R, S, g, staged, kick);
+    force_part<PREC, SF>(B, R, S, g, gb, staged, kick);
   } else {
     each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
       const Brick brick = B.bricks[gb];
@@ -1629,7 +1669,7 @@ __global__ void __launch_bounds__(L::kThreads) k_force(Buffers B, int kick) {
       const Brick g = part_brick(brick, code, R);
       __syncthreads();  // the previous part is done with S
       const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-      force_part<PREC, SF>(B, R, S, g, staged, kick);
+      force_part<PREC, SF>(B, R, S, g, gb, staged, kick);
     });
   }
 }
@@ -2325,12 +2365,18 @@ hipError_t launch_decide(const Buffers& B, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_drift(const Buffers& B, bool langevin, hipStream_t s) {
+hipError_t launch_drift(const Buffers& B, bool langevin, bool keep_v, hipStream_t s) {
   if (!B.n_atom_blocks) return hipSuccess;
   if (langevin)
-    hipLaunchKernelGGL(k_drift<true>, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL(k_drift<true>, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B, keep_v ? 1 : 0);
   else
-    hipLaunchKernelGGL(k_drift<false>, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL(k_drift<false>, grid1(dealt_grid(B.n_atom_blocks)), dim3(kBlock), 0, s, B, keep_v ? 1 : 0);
+  hipLaunchKernelGGL(k_decide, decide_grid(B), dim3(kBlock), 0, s, B, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_decide_step(const Buffers& B, hipStream_t s) {
+  if (!B.n_atom_blocks) return hipSuccess;
   hipLaunchKernelGGL(k_decide, decide_grid(B), dim3(kBlock), 0, s, B, 1);
   return hipGetLastError();
 }
@@ -2470,9 +2516,9 @@ hipError_t force_launches(const Buffers& B, int k, hipStream_t s, const Side* si
   return e;
 }
 
-hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s, const Side* side) {
+hipError_t launch_force(const Buffers& B, int precision, bool shifted, int kick, hipStream_t s, const Side* side) {
   if (!B.n_atom_blocks) return hipSuccess;
-  const int k = kick ? 1 : 0;
+  const int k = kick;
   hipError_t e;
   if (precision == 0)
     e = shifted ? force_launches<0, true>(B, k, s, side) : force_launches<0, false>(B, k, s, side);

@@ -25,7 +25,12 @@ struct TuneTrial {
 };
 
 hipError_t launch_decide(const Buffers& B, hipStream_t s);
-hipError_t launch_drift(const Buffers& B, bool langevin, hipStream_t s);
+// The drift and the step's rebuild decision; keep_v: positions only (the
+// drift-ahead form's first step).
+hipError_t launch_drift(const Buffers& B, bool langevin, bool keep_v, hipStream_t s);
+// The step's rebuild decision alone (the drift-ahead form, whose force
+// launch of the step before drifted the positions).
+hipError_t launch_decide_step(const Buffers& B, hipStream_t s);
 hipError_t launch_rebuild(const Buffers& B, hipStream_t s, const Side* side = nullptr);
 // The lists of every replica again from the positions of the last build
 // (xb), for a new brick table; the cut marks of the table must start at
@@ -38,7 +43,10 @@ hipError_t launch_nh_half(const Buffers& B, bool scale, hipStream_t s);
 hipError_t launch_heat_exchange(const Buffers& B, hipStream_t s);
 // One heat-current sample into sample `sample` of the series.
 hipError_t launch_heat_current(const Buffers& B, bool shifted, uint32_t sample, hipStream_t s);
-hipError_t launch_force(const Buffers& B, int precision, bool shifted, bool kick, hipStream_t s,
+// kick 0: accelerations only; 1: and the second half kick; 2: the
+// drift-ahead form (NVE): the whole velocity step and the next step's drift
+// into the second position set.
+hipError_t launch_force(const Buffers& B, int precision, bool shifted, int kick, hipStream_t s,
                         const Side* side = nullptr);
 hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStream_t s);
 // wide: every replica's profile slabs are at least a cutoff wide.

@@ -93,7 +93,16 @@ struct vlj_sim {
   Buffers B{};
   std::vector<void*> allocations;
   Ring ring[kRing];
-  hipGraphExec_t graph1 = nullptr, graphN = nullptr;
+  // captured steps, for each orientation of the position sets
+  hipGraphExec_t graph1[2] = {nullptr, nullptr}, graphN[2] = {nullptr, nullptr};
+  // The drift-ahead form (every replica NVE): a step's force launch also
+  // drifts the positions of the next step into the second position set
+  // (B.x2); `ahead` says it holds them. Each step that uses them swaps the
+  // sets (and the rebuild words), so between calls B.x always holds the
+  // positions of the present state. `orient`: whether the sets are swapped
+  // from their allocation.
+  bool ahead = false;
+  int orient = 0;
   bool primed = false;
   uint64_t steps = 0;
   long long* fixed_scratch = nullptr;
@@ -322,8 +331,10 @@ void free_device(vlj_sim* s, void* p) {
 
 void free_all(vlj_sim* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  if (s->graph1) (void)hipGraphExecDestroy(s->graph1);
-  if (s->graphN) (void)hipGraphExecDestroy(s->graphN);
+  for (int k = 0; k < 2; k++) {
+    if (s->graph1[k]) (void)hipGraphExecDestroy(s->graph1[k]);
+    if (s->graphN[k]) (void)hipGraphExecDestroy(s->graphN[k]);
+  }
   for (auto& r : s->ring) {
     if (r.slots) (void)hipHostFree(r.slots);
     if (r.ctl) (void)hipHostFree(r.ctl);
@@ -366,13 +377,34 @@ int check_device(vlj_sim* s) {
 // one after the force kick.
 // `around_force`, when given, is a pair of events recorded before and after
 // the pair-force launches.
+// Whether the steps take the drift-ahead form: every replica NVE (a
+// thermostat or heat exchange acts between the half steps).
+bool drift_ahead(const vlj_sim* s) { return !s->any_langevin && !s->any_nose_hoover && !s->any_heat_exchange; }
+
+// Swaps the two position sets and the two rebuild words.
+void flip_positions(vlj_sim* s) {
+  Buffers& B = s->B;
+  std::swap(B.x, B.x2);
+  std::swap(B.y, B.y2);
+  std::swap(B.z, B.z2);
+  std::swap(B.rebuilding, B.rebuilding_next);
+  s->orient ^= 1;
+}
+
 int enqueue_step(vlj_sim* s, hipEvent_t* around_force = nullptr) {
+  const bool fused = drift_ahead(s);
   if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, false, s->stream));
-  HIP_TRY(s, launch_drift(s->B, s->any_langevin, s->stream));
+  if (fused && s->ahead) {
+    flip_positions(s);  // the positions the last force launch drifted
+    HIP_TRY(s, launch_decide_step(s->B, s->stream));
+  } else {
+    HIP_TRY(s, launch_drift(s->B, s->any_langevin, fused, s->stream));
+  }
   HIP_TRY(s, launch_rebuild(s->B, s->stream, &s->side));
   if (around_force) HIP_TRY(s, hipEventRecord(around_force[0], s->stream));
-  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, true, s->stream, &s->side));
+  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, fused ? 2 : 1, s->stream, &s->side));
   if (around_force) HIP_TRY(s, hipEventRecord(around_force[1], s->stream));
+  s->ahead = fused;
   if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, true, s->stream));
   if (s->any_heat_exchange) HIP_TRY(s, launch_heat_exchange(s->B, s->stream));
   return VLJ_OK;
@@ -382,16 +414,26 @@ int enqueue_step(vlj_sim* s, hipEvent_t* around_force = nullptr) {
 // on the next step.
 void drop_graphs(vlj_sim* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  if (s->graph1) (void)hipGraphExecDestroy(s->graph1);
-  if (s->graphN) (void)hipGraphExecDestroy(s->graphN);
-  s->graph1 = s->graphN = nullptr;
+  for (int k = 0; k < 2; k++) {
+    if (s->graph1[k]) (void)hipGraphExecDestroy(s->graph1[k]);
+    if (s->graphN[k]) (void)hipGraphExecDestroy(s->graphN[k]);
+    s->graph1[k] = s->graphN[k] = nullptr;
+  }
 }
 
+// Captures `steps` steps from the present state; the handle's host state
+// (position sets, drift-ahead flag) is left as before, since nothing ran.
 int capture(vlj_sim* s, uint32_t steps, hipGraphExec_t* out) {
   hipGraph_t g = nullptr;
+  const Buffers saved = s->B;
+  const bool ahead = s->ahead;
+  const int orient = s->orient;
   HIP_TRY(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = VLJ_OK;
   for (uint32_t t = 0; t < steps && rc == VLJ_OK; t++) rc = enqueue_step(s);
+  s->B = saved;
+  s->ahead = ahead;
+  s->orient = orient;
   hipError_t e = hipStreamEndCapture(s->stream, &g);
   if (rc != VLJ_OK) {
     if (g) (void)hipGraphDestroy(g);
@@ -595,7 +637,7 @@ int prime_if_needed(vlj_sim* s) {
   if (s->primed) return VLJ_OK;
   HIP_TRY(s, launch_decide(s->B, s->stream));
   HIP_TRY(s, launch_rebuild(s->B, s->stream, &s->side));
-  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, false, s->stream, &s->side));
+  HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, 0, s->stream, &s->side));
   s->primed = true;
   return VLJ_OK;
 }
@@ -960,7 +1002,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   A(&B.slots, replicas * kSlots);
   A(&B.profile, std::max<uint64_t>(s->prof_total, 1));
   A(&B.tickets, 4);
-  A(&B.flags, 2);
+  A(&B.flags, 3);
   A(&s->fixed_scratch, kSlots);
   if (rc != VLJ_OK) {
     std::string msg = s->error;
@@ -1013,7 +1055,9 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   if (e == hipSuccess) e = hipMemset(B.cell_count, 0, std::max<size_t>(C, 1) * 4);
   if (e == hipSuccess) e = hipMemset(B.cell_cursor, 0, std::max<size_t>(C, 1) * 4);
   if (e == hipSuccess) e = hipMemset(B.tickets, 0, 16);
-  if (e == hipSuccess) e = hipMemset(B.flags, 0, 8);
+  if (e == hipSuccess) e = hipMemset(B.flags, 0, 12);
+  B.rebuilding = B.flags;
+  B.rebuilding_next = B.flags + 2;
   if (e == hipSuccess) e = hipMemset(B.ax, 0, std::max<size_t>(N, 1) * 8);
   if (e == hipSuccess) e = hipMemset(B.ay, 0, std::max<size_t>(N, 1) * 8);
   if (e == hipSuccess) e = hipMemset(B.az, 0, std::max<size_t>(N, 1) * 8);
@@ -1035,10 +1079,13 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   return VLJ_OK;
 }
 
+// The captured steps for the present orientation of the position sets (the
+// drift-ahead form swaps them every step: one pair of graphs each way).
 int ensure_graphs(vlj_sim* s) {
-  if (s->graph1) return VLJ_OK;
-  int rc = capture(s, 1, &s->graph1);
-  if (rc == VLJ_OK) rc = capture(s, kGraphSteps, &s->graphN);
+  const int o = s->orient;
+  int rc = VLJ_OK;
+  if (!s->graph1[o]) rc = capture(s, 1, &s->graph1[o]);
+  if (rc == VLJ_OK && !s->graphN[o]) rc = capture(s, kGraphSteps, &s->graphN[o]);
   return rc;
 }
 
@@ -1057,17 +1104,30 @@ int enqueue_steps(vlj_sim* s, uint64_t steps) {
   }
   rc = check_bricks(s);
   if (rc != VLJ_OK) return rc;
-  rc = ensure_graphs(s);
-  if (rc != VLJ_OK) return rc;
   uint64_t left = steps;
+  // The captured drift-ahead steps start from drifted positions: the first
+  // step after a change of form drifts on its own launch.
+  if (drift_ahead(s) && !s->ahead && left > 0) {
+    rc = enqueue_step(s);
+    if (rc != VLJ_OK) return rc;
+    left--;
+  }
   while (left >= kGraphSteps) {
-    HIP_TRY(s, hipGraphLaunch(s->graphN, s->stream));
+    rc = ensure_graphs(s);
+    if (rc != VLJ_OK) return rc;
+    HIP_TRY(s, hipGraphLaunch(s->graphN[s->orient], s->stream));  // an even count: the sets end as they began
     left -= kGraphSteps;
   }
   while (left > 0) {
-    HIP_TRY(s, hipGraphLaunch(s->graph1, s->stream));
+    rc = ensure_graphs(s);
+    if (rc != VLJ_OK) return rc;
+    HIP_TRY(s, hipGraphLaunch(s->graph1[s->orient], s->stream));
+    if (drift_ahead(s)) flip_positions(s);
     left--;
   }
+  // what the replayed steps leave: drifted positions in B.x2 after
+  // drift-ahead steps, none after the others (which use B.x2 as scratch)
+  if (steps > 0) s->ahead = drift_ahead(s);
   s->steps += steps;
   return check_bricks(s);
 }
@@ -1858,12 +1918,10 @@ int vlj_timed_run(vlj_sim* sim, uint64_t steps, double* force_ms, double* total_
   std::vector<hipEvent_t> ev(2 * steps + 2, nullptr);
   for (auto& e : ev) HIP_TRY(sim, hipEventCreate(&e));
   HIP_TRY(sim, hipEventRecord(ev[2 * steps], sim->stream));
-  for (uint64_t t = 0; t < steps && rc == VLJ_OK; t++) {
-    HIP_TRY(sim, launch_drift(sim->B, sim->any_langevin, sim->stream));
-    HIP_TRY(sim, launch_rebuild(sim->B, sim->stream));
-    HIP_TRY(sim, hipEventRecord(ev[2 * t], sim->stream));
-    HIP_TRY(sim, launch_force(sim->B, sim->precision, sim->shifted, true, sim->stream));
-    HIP_TRY(sim, hipEventRecord(ev[2 * t + 1], sim->stream));
+  for (uint64_t t = 0; t < steps && rc == VLJ_OK; t++) rc = enqueue_step(sim, &ev[2 * t]);
+  if (rc != VLJ_OK) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
   }
   HIP_TRY(sim, hipEventRecord(ev[2 * steps + 1], sim->stream));
   HIP_TRY(sim, hipEventSynchronize(ev[2 * steps + 1]));
