@@ -1363,6 +1363,7 @@ static int set_temperature(vlj_sim* sim, size_t replica, double temperature) {
   HIP_TRY(sim, hipMemcpyAsync((void*)&sim->B.rep[replica], &R, sizeof(Replica), hipMemcpyHostToDevice,
                               sim->stream));
   HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+  sim->ahead = false;  // a parameter the next drift uses changed
   return VLJ_OK;
 }
 
@@ -1486,6 +1487,7 @@ int vlj_set_heat_exchange(vlj_sim* sim, int32_t on, const double hot[2], const d
     drop_graphs(sim);
     sim->any_heat_exchange = any;
   }
+  sim->ahead = false;
   return VLJ_OK;
 }
 
