@@ -1910,6 +1910,134 @@ __device__ __forceinline__ void profile_part(const Buffers& B, const Replica& R,
   }
 }
 
+// A pair's potential u and q = rF / r^2 (R10, FP64 in both forms).
+template <bool SF>
+__device__ __forceinline__ void profile_pair_terms(const Replica& R, double r2, double& u, double& q) {
+  const double inv2 = rcp_rn(r2);  // = 1.0 / r2, correctly rounded
+  const double s6 = inv2 * inv2 * inv2;
+  double rf = s6 * __builtin_fma(48.0, s6, -24.0);
+  const double t = 4.0 * s6;
+  u = __builtin_fma(t, s6, -t) - R.shift;
+  if (SF) {
+    const double r = __builtin_sqrt(r2);
+    rf = __builtin_fma(-r, R.force_shift, rf);
+    u = __builtin_fma(r - R.rc, R.force_shift, u);
+  }
+  q = rf * inv2;
+}
+
+// A pair's virial half w spread over the slabs its segment from z (wrapped,
+// in slab hb) to z + dz crosses, in proportion to the length inside each,
+// the last slab taking what is left (R10): into the atom's window sums, or
+// to the profile at once for a slab outside them.
+__device__ __forceinline__ void walk_shares(double zw, double dz, int hb, int home, double width, double iw,
+                                            uint32_t bins, const double w[3], SlabWindow& win,
+                                            const ProfileSink& sink) {
+  const int ibins = (int)bins;
+  const double lo = zw + (dz < 0.0 ? dz : 0.0);
+  const double hi = zw + (dz > 0.0 ? dz : 0.0);
+  const double length = hi - lo;
+  if (length < 1e-6 * width) {
+    win.add(home - hb, home, ibins, w, 1.0, sink);
+    return;
+  }
+  int b = (int)floor(lo * iw);
+  double left = 1.0;
+  for (uint32_t walk = 0; walk < bins + 2u; walk++) {
+    const bool last = (double)(b + 1) * width >= hi;
+    const double start = lo > (double)b * width ? lo : (double)b * width;
+    const double end = hi < (double)(b + 1) * width ? hi : (double)(b + 1) * width;
+    double share = last ? left : (end - start) / length;
+    share = share > 0.0 ? share : 0.0;
+    left -= share;
+    win.add(b - hb, b, ibins, w, share, sink);
+    if (last) break;
+    b++;
+  }
+}
+
+// One atom's pairs for slabs at least a cutoff wide, from the brick's LDS:
+// the list in whole octets (pads meet the far sentinel, outside the cutoff),
+// every pair through the same operations without a branch. A segment no
+// longer than a slab lies in the atom's slab or crosses one slab edge, into
+// the slab below or above; the walk above gives its shares as
+//   s1 = (end - start) / length (clamped at 0) to the lower slab b and
+//   s2 = 1 - s1 (clamped) to b + 1, or 1 to b when the segment ends in b,
+// so each pair adds w s to the atom's own slab and, when it crosses, to the
+// slab below or above; adds of pairs outside the cutoff are w 0 (w is
+// finite for them, the sentinel's included). A pair the two-slab form does
+// not cover (a segment ending exactly on an edge below the atom, a tiny
+// segment of an atom whose slab was clamped into the box) takes the walk.
+template <bool SF, class L>
+__device__ __forceinline__ void wide_pairs_staged(const Replica& R, const Stage<L>& S, const uint16_t* lst,
+                                                  uint32_t cnt, double xi, double yi, double zi, double zw, int hb,
+                                                  int home, double width, double iw, uint32_t bins,
+                                                  const ProfileSink& sink, double& pe, double total[3],
+                                                  SlabWindow& win) {
+  const double tiny_length = 1e-6 * width;
+  const uint32_t no = (cnt + 7u) / 8u;
+  const uint32_t last_octet = no > 0u ? no - 1u : 0u;
+  u32x4 cur = load_octet(lst, 0u, last_octet);
+  for (uint32_t o = 0; o < no; o++) {
+    const u32x4 nxt = load_octet(lst, o + 1u, last_octet);
+    uint32_t w0 = cur.x, w1 = cur.y, w2 = cur.z, w3 = cur.w;
+    // one pair at a time (the pair body is long): the octet's entries shift
+    // down 16 bits a pair
+#pragma unroll 1
+    for (int c = 0; c < 8; c++) {
+      const uint32_t off = w0 & 0xffffu;  // 8 e
+      w0 = (w0 >> 16) | (w1 << 16);
+      w1 = (w1 >> 16) | (w2 << 16);
+      w2 = (w2 >> 16) | (w3 << 16);
+      w3 >>= 16;
+      const double d[3] = {*(const double*)((const char*)S.x + off) - xi,
+                           *(const double*)((const char*)S.y + off) - yi,
+                           *(const double*)((const char*)S.z + off) - zi};
+      const double r2 = dist2(d[0], d[1], d[2]);
+      const bool inside = r2 < R.rc2;
+      double u, q;
+      profile_pair_terms<SF>(R, r2, u, q);
+      const double sel = inside ? 1.0 : 0.0;
+      pe = __builtin_fma(0.5 * u, sel, pe);
+      double w[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        w[a] = (0.5 * d[a]) * d[a] * q;
+        total[a] = __builtin_fma(w[a], sel, total[a]);
+      }
+      const double lo = zw + (d[2] < 0.0 ? d[2] : 0.0);
+      const double hi = zw + (d[2] > 0.0 ? d[2] : 0.0);
+      const double length = hi - lo;
+      const bool tiny = length < tiny_length;
+      const int b = (int)floor(lo * iw);
+      const double bw = (double)b * width, e1 = (double)(b + 1) * width;
+      const bool last1 = e1 >= hi;
+      const bool last2 = (double)(b + 2) * width >= hi;
+      const double start = lo > bw ? lo : bw;
+      const double end = hi < e1 ? hi : e1;
+      double s1 = last1 ? 1.0 : (end - start) / length;
+      s1 = s1 > 0.0 ? s1 : 0.0;
+      double s2 = 1.0 - s1;
+      s2 = s2 > 0.0 ? s2 : 0.0;
+      const int rel = b - hb;
+      const bool up = !tiny && !last1 && rel == 0, dn = !tiny && !last1 && rel == -1;
+      const bool simple = tiny ? home == hb : (last1 ? rel == 0 : last2 && (up || dn));
+      const bool take = inside && simple;
+      const double so = take ? (tiny ? 1.0 : (dn ? s2 : s1)) : 0.0;
+      const double sx = take ? (up ? s2 : s1) : 0.0;
+      const double su = up ? sx : 0.0, sd = dn ? sx : 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        win.w[0][a] += w[a] * sd;
+        win.w[1][a] += w[a] * so;
+        win.w[2][a] += w[a] * su;
+      }
+      if (inside && !simple) walk_shares(zw, d[2], hb, home, width, iw, bins, w, win, sink);
+    }
+    cur = nxt;
+  }
+}
+
 // Slabs at least a cutoff wide: a wave's atoms put all their items in the
 // slabs hmin - 1 .. hmin + kWaveSlots - 2, hmin the lowest home slab in the
 // wave (a wave's home atoms lie in a few consecutive cells of one or two
@@ -1993,51 +2121,28 @@ __device__ __forceinline__ void profile_part_wide(const Buffers& B, const Replic
       double pe = 0.0, total[3] = {0.0, 0.0, 0.0};
       const uint32_t cnt = B.nbr_count[i];
       const uint16_t* lst = B.nbr + list_base(R, i - R.atom_off);
-      each_entry(lst, cnt, [&](uint32_t k, uint32_t entry) {
-        double xj, yj, zj;
-        staged_pos(R, S, staged, list_index(entry, staged), B.x, B.y, B.z, xj, yj, zj);
-        const double d[3] = {xj - xi, yj - yi, zj - zi};
-        const double r2 = dist2(d[0], d[1], d[2]);
-        if (!(r2 < R.rc2)) return;
-        const double inv2 = rcp_rn(r2);  // = 1.0 / r2, correctly rounded
-        const double s6 = inv2 * inv2 * inv2;
-        double rf = s6 * __builtin_fma(48.0, s6, -24.0);
-        const double t = 4.0 * s6;
-        double u = __builtin_fma(t, s6, -t) - R.shift;
-        if (SF) {
-          const double r = __builtin_sqrt(r2);
-          rf = __builtin_fma(-r, R.force_shift, rf);
-          u = __builtin_fma(r - R.rc, R.force_shift, u);
-        }
-        pe += 0.5 * u;
-        const double q = rf * inv2;
-        double w[3];
+      if (staged) {
+        wide_pairs_staged<SF>(R, S, lst, cnt, xi, yi, zi, zw, hb, (int)home, width, iw, bins, direct, pe, total,
+                              win);
+      } else {
+        each_entry(lst, cnt, [&](uint32_t k, uint32_t entry) {
+          double xj, yj, zj;
+          staged_pos(R, S, false, list_index(entry, false), B.x, B.y, B.z, xj, yj, zj);
+          const double d[3] = {xj - xi, yj - yi, zj - zi};
+          const double r2 = dist2(d[0], d[1], d[2]);
+          if (!(r2 < R.rc2)) return;
+          double u, q;
+          profile_pair_terms<SF>(R, r2, u, q);
+          pe += 0.5 * u;
+          double w[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-          w[a] = (0.5 * d[a]) * d[a] * q;
-          total[a] += w[a];
-        }
-        const double lo = zw + (d[2] < 0.0 ? d[2] : 0.0);
-        const double hi = zw + (d[2] > 0.0 ? d[2] : 0.0);
-        const double length = hi - lo;
-        if (length < 1e-6 * width) {
-          win.add((int)home - hb, (int)home, ibins, w, 1.0, direct);
-          return;
-        }
-        int b = (int)floor(lo * iw);
-        double left = 1.0;
-        for (uint32_t walk = 0; walk < bins + 2u; walk++) {
-          const bool last = (double)(b + 1) * width >= hi;
-          const double start = lo > (double)b * width ? lo : (double)b * width;
-          const double end = hi < (double)(b + 1) * width ? hi : (double)(b + 1) * width;
-          double share = last ? left : (end - start) / length;
-          share = share > 0.0 ? share : 0.0;
-          left -= share;
-          win.add(b - hb, b, ibins, w, share, direct);
-          if (last) break;
-          b++;
-        }
-      });
+          for (int a = 0; a < 3; a++) {
+            w[a] = (0.5 * d[a]) * d[a] * q;
+            total[a] += w[a];
+          }
+          walk_shares(zw, d[2], hb, (int)home, width, iw, bins, w, win, direct);
+        });
+      }
       pe_fixed = to_fixed(pe, kProfileOne);
 #pragma unroll
       for (int a = 0; a < 3; a++) whole[a] += to_fixed(total[a], kProfileOne);
