@@ -126,7 +126,7 @@ struct Control {
   uint32_t rebuilds;
   uint32_t overflow;
   uint32_t built;
-  uint32_t stage_overflow;        // brick list builds that cut the brick into parts
+  uint32_t stage_overflow;        // brick list builds whose neighbourhood outgrew LDS
   uint32_t stage_global;          // ... and parts of one home cell that read global memory
   uint32_t pad_;
   double xi, eta, sm1;            // Nose-Hoover friction, its integral, velocity scale - 1
@@ -136,11 +136,9 @@ struct Range {
   uint32_t rep, first, end, pad;
 };
 
-// A brick: gx x gy home cells of one z layer of a replica's grid, from the
-// cell (x0, y0, z) = cell0 (index within the replica). Its workgroup stages
-// the 3 x (gy + 2) x (gx + 2) cells around it.
-// gx x gy x gz home cells from cell0 (its lowest x, y and z) of replica
-// rep; gz is 1 or 2 cell layers.
+// A brick: gx x gy x gz home cells from cell0 (its lowest x, y and z, an
+// index within the replica's grid) of replica rep; gz is 1 or 2 cell layers.
+// Its workgroup stages the (gz + 2) x (gy + 2) x (gx + 2) cells around it.
 struct Brick {
   uint32_t rep, cell0, gx, gy, gz;
 };
@@ -158,6 +156,10 @@ struct Buffers {
   uint16_t* parts;            // per brick: kHomeCells packed parts
   uint32_t* cut_groups;       // per group of kCutGroup bricks: bricks cut
   uint32_t n_atom_blocks, n_scan_blocks, n_sort_blocks, n_bricks, n_replicas, n_dense;
+  // 1: a brick whose neighbourhood outgrows LDS is cut into parts, which the
+  // PARTS launches after each brick launch take; 0: no PARTS launches, and
+  // such a brick runs whole, its positions read through the cell tables
+  uint32_t parts_on;
   unsigned long long* bmax;   // per atom block: max moved^2 (FP64 bits)
   long long* kepart;          // per atom block: kinetic energy in fixed point (Nose-Hoover)
   long long* hexpart;         // per atom block: kHexSums heat-exchange sums

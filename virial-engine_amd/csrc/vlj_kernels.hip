@@ -1306,7 +1306,11 @@ __global__ void __launch_bounds__(L::kThreads) k_build(Buffers B, int relist) {
     const double* Z = relist ? B.zb : (moved ? B.z2 : B.z);
     if (threadIdx.x == 0) next_item = 0u;
     const bool staged = stage_brick(B, R, g, X, Y, Z, S);  // ends with a barrier
-    const bool cut = !staged && home_cells(g) > 1u;
+    // A neighbourhood that does not fit is cut into parts when the handle
+    // runs the PARTS launches (parts_on), else the brick is built whole from
+    // global memory; either way the overflow is counted.
+    const bool over = !staged && home_cells(g) > 1u;
+    const bool cut = over && B.parts_on;
     if (threadIdx.x == 0) {  // mark the brick and record its parts for the PARTS launches
       const uint32_t was = B.cut[gb];
       const uint32_t np = cut ? cut_brick(g, S, B.parts + (size_t)gb * kHomeCells) : 0u;
@@ -1315,7 +1319,7 @@ __global__ void __launch_bounds__(L::kThreads) k_build(Buffers B, int relist) {
         atomicAdd(&B.cut_groups[gb / kCutGroup], np != 0u ? 1u : 0xffffffffu);
         atomicAdd(&B.flags[1], np != 0u ? 1u : 0xffffffffu);
       }
-      if (cut && !relist) atomicAdd(&B.ctl[g.rep].stage_overflow, 1u);
+      if (over && !relist) atomicAdd(&B.ctl[g.rep].stage_overflow, 1u);
     }
     if (cut) return;
     build_part(B, R, S, g, staged, moved, relist != 0, X, Y, Z, next_item);
@@ -1660,7 +1664,7 @@ __global__ void __launch_bounds__(L::kThreads) k_force(Buffers B, int kick) {
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-    if (!staged && home_cells(g) > 1u) return;
+    if (!staged && home_cells(g) > 1u && B.parts_on) return;  // cut into parts
     force_part<PREC, SF>(B, R, S, g, gb, staged, kick);
   } else {
     each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
@@ -1741,7 +1745,7 @@ __global__ void __launch_bounds__(L::kThreads) k_measure(Buffers B) {
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-    if (!staged && home_cells(g) > 1u) return;
+    if (!staged && home_cells(g) > 1u && B.parts_on) return;  // cut into parts
     measure_part<PREC, SF>(B, R, S, g, staged, B.slots + (size_t)g.rep * kSlots);
   } else {
     each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
@@ -2201,7 +2205,7 @@ __global__ void __launch_bounds__(L::kThreads, WIDE && !PARTS && !L::kSparse ? 4
     const Brick g = PARTS ? part_brick(brick, code, R) : brick;
     if (PARTS) __syncthreads();  // the previous part is done with S
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
-    if (!PARTS && !staged && home_cells(g) > 1u) return;
+    if (!PARTS && !staged && home_cells(g) > 1u && B.parts_on) return;  // cut into parts
     long long whole[3] = {0, 0, 0};
     long long* prof = B.profile + R.prof_off;
     profile_part_wide<SF>(B, R, S, g, staged, prof, whole);
@@ -2234,7 +2238,7 @@ __global__ void __launch_bounds__(L::kThreads, WIDE && !PARTS && !L::kSparse ? 4
     if (sink.use_lds)
       for (uint32_t k = threadIdx.x; k < nsums; k += L::kThreads) acc[k] = 0;
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);  // ends with a barrier
-    if (!PARTS && !staged && home_cells(g) > 1u) return;
+    if (!PARTS && !staged && home_cells(g) > 1u && B.parts_on) return;  // cut into parts
     long long whole[3] = {0, 0, 0};
     profile_part<SF>(B, R, S, g, staged, sink, whole);
     profile_flush<L>(sink, bins, whole);
@@ -2308,7 +2312,7 @@ __global__ void __launch_bounds__(L::kThreads) k_heat_current(Buffers B, uint32_
     const Replica R = B.rep[g.rep];
     if (R.hc_cap == 0u || sample >= R.hc_cap) return;
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-    if (!staged && home_cells(g) > 1u) return;
+    if (!staged && home_cells(g) > 1u && B.parts_on) return;  // cut into parts
     long long jsum[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
     heat_part<SF>(B, R, S, g, staged, jsum);
     block_add_slots<3, L::kThreads>(jsum, B.hc + 3ull * sample);
@@ -2381,7 +2385,7 @@ __global__ void __launch_bounds__(L::kThreads) k_render(Buffers B) {
     const Brick g = B.bricks[gb];
     const Replica R = B.rep[g.rep];
     const bool staged = stage_brick(B, R, g, B.x, B.y, B.z, S);
-    if (!staged && home_cells(g) > 1u) return;
+    if (!staged && home_cells(g) > 1u && B.parts_on) return;  // cut into parts
     render_part(B, R, S, g, staged);
   } else {
     each_cut_part<L>(B, [&](uint32_t gb, uint32_t code) {
@@ -2523,9 +2527,9 @@ inline dim3 brick_grid(const Buffers& B) {
       hipLaunchKernelGGL((KERNEL<Dense, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, __VA_ARGS__); \
     if (layout_count<Sparse>(B))                                                                          \
       hipLaunchKernelGGL((KERNEL<Sparse, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, __VA_ARGS__); \
-    if (layout_count<Dense>(B))                                                                           \
+    if (B.parts_on && layout_count<Dense>(B))                                                             \
       hipLaunchKernelGGL((KERNEL<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, __VA_ARGS__);  \
-    if (layout_count<Sparse>(B))                                                                          \
+    if (B.parts_on && layout_count<Sparse>(B))                                                            \
       hipLaunchKernelGGL((KERNEL<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, __VA_ARGS__); \
   } while (0)
 
@@ -2545,8 +2549,8 @@ void heat_launches(const Buffers& B, uint32_t sample, hipStream_t s) {
     hipLaunchKernelGGL((k_heat_current<Sparse, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B,
                        sample);
   }
-  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_heat_current<Dense, SF, true>), gp, dim3(Dense::kThreads), 0, s, B, sample);
-  if (layout_count<Sparse>(B))
+  if (B.parts_on && layout_count<Dense>(B)) hipLaunchKernelGGL((k_heat_current<Dense, SF, true>), gp, dim3(Dense::kThreads), 0, s, B, sample);
+  if (B.parts_on && layout_count<Sparse>(B))
     hipLaunchKernelGGL((k_heat_current<Sparse, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B, sample);
 }
 
@@ -2590,8 +2594,8 @@ hipError_t launch_rebuild(const Buffers& B, hipStream_t s, const Side* side) {
   if (layout_count<Sparse>(B))
     hipLaunchKernelGGL((k_build<Sparse, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, ss, B, 0);
   join_side(s, ss, side, e);
-  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_build<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, B, 0);
-  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_build<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, B, 0);
+  if (B.parts_on && layout_count<Dense>(B)) hipLaunchKernelGGL((k_build<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, B, 0);
+  if (B.parts_on && layout_count<Sparse>(B)) hipLaunchKernelGGL((k_build<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, B, 0);
   return e != hipSuccess ? e : hipGetLastError();
 }
 
@@ -2601,8 +2605,8 @@ hipError_t launch_relist(const Buffers& B, hipStream_t s) {
     hipLaunchKernelGGL((k_build<Dense, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B, 1);
   if (layout_count<Sparse>(B))
     hipLaunchKernelGGL((k_build<Sparse, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B, 1);
-  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_build<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, B, 1);
-  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_build<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, B, 1);
+  if (B.parts_on && layout_count<Dense>(B)) hipLaunchKernelGGL((k_build<Dense, true>), parts_grid(B), dim3(Dense::kThreads), 0, s, B, 1);
+  if (B.parts_on && layout_count<Sparse>(B)) hipLaunchKernelGGL((k_build<Sparse, true>), parts_grid(B), dim3(Sparse::kThreads), 0, s, B, 1);
   return hipGetLastError();
 }
 
@@ -2616,8 +2620,8 @@ hipError_t force_launches(const Buffers& B, int k, hipStream_t s, const Side* si
   if (layout_count<Sparse>(B))
     hipLaunchKernelGGL((k_force<Sparse, PREC, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, ss, B, k);
   join_side(s, ss, side, e);
-  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_force<Dense, PREC, SF, true>), gp, dim3(Dense::kThreads), 0, s, B, k);
-  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_force<Sparse, PREC, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B, k);
+  if (B.parts_on && layout_count<Dense>(B)) hipLaunchKernelGGL((k_force<Dense, PREC, SF, true>), gp, dim3(Dense::kThreads), 0, s, B, k);
+  if (B.parts_on && layout_count<Sparse>(B)) hipLaunchKernelGGL((k_force<Sparse, PREC, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B, k);
   return e;
 }
 
@@ -2639,8 +2643,8 @@ void measure_launches(const Buffers& B, hipStream_t s) {
     hipLaunchKernelGGL((k_measure<Dense, PREC, SF, false>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B);
   if (layout_count<Sparse>(B))
     hipLaunchKernelGGL((k_measure<Sparse, PREC, SF, false>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B);
-  if (layout_count<Dense>(B)) hipLaunchKernelGGL((k_measure<Dense, PREC, SF, true>), gp, dim3(Dense::kThreads), 0, s, B);
-  if (layout_count<Sparse>(B)) hipLaunchKernelGGL((k_measure<Sparse, PREC, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B);
+  if (B.parts_on && layout_count<Dense>(B)) hipLaunchKernelGGL((k_measure<Dense, PREC, SF, true>), gp, dim3(Dense::kThreads), 0, s, B);
+  if (B.parts_on && layout_count<Sparse>(B)) hipLaunchKernelGGL((k_measure<Sparse, PREC, SF, true>), gp, dim3(Sparse::kThreads), 0, s, B);
 }
 
 hipError_t launch_measure(const Buffers& B, int precision, bool shifted, hipStream_t s) {
@@ -2668,9 +2672,9 @@ void profile_launches(const Buffers& B, hipStream_t s) {
     hipLaunchKernelGGL((k_profile<Dense, SF, false, WIDE>), brick_grid<Dense>(B), dim3(Dense::kThreads), 0, s, B);
   if (layout_count<Sparse>(B))
     hipLaunchKernelGGL((k_profile<Sparse, SF, false, WIDE>), brick_grid<Sparse>(B), dim3(Sparse::kThreads), 0, s, B);
-  if (layout_count<Dense>(B))
+  if (B.parts_on && layout_count<Dense>(B))
     hipLaunchKernelGGL((k_profile<Dense, SF, true, WIDE>), gp, dim3(Dense::kThreads), 0, s, B);
-  if (layout_count<Sparse>(B))
+  if (B.parts_on && layout_count<Sparse>(B))
     hipLaunchKernelGGL((k_profile<Sparse, SF, true, WIDE>), gp, dim3(Sparse::kThreads), 0, s, B);
 }
 

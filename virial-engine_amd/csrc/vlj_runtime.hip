@@ -825,6 +825,11 @@ int check_bricks(vlj_sim* s) {
       std::vector<uint32_t> cn(s->cells_total);
       HIP_TRY(s, hipStreamSynchronize(s->stream));
       HIP_TRY(s, hipMemcpy(cn.data(), s->B.cell_n, s->cells_total * 4, hipMemcpyDeviceToHost));
+      // A handle without the PARTS launches ran its outgrown bricks whole
+      // from global memory; from now on they are cut into parts, which needs
+      // the lists of those replicas built again.
+      const bool parts_now = s->B.parts_on == 0u;
+      s->B.parts_on = 1u;
       any = false;
       for (size_t r = 0; r < R; r++) {
         if (!which[r]) continue;
@@ -837,7 +842,7 @@ int check_bricks(vlj_sim* s) {
         std::vector<uint32_t> bx, by;
         std::vector<uint8_t> bz, sp;
         choose_bricks(rp, occ, plan_fill(s) * H.headroom, plan_sparse_fill(s) * H.headroom, bx, by, bz, sp);
-        which[r] = bx != H.brick_x || by != H.brick_y || bz != H.brick_z || sp != H.sparse;
+        which[r] = parts_now || bx != H.brick_x || by != H.brick_y || bz != H.brick_z || sp != H.sparse;
         any = any || which[r];
       }
       if (any) {
@@ -854,6 +859,32 @@ int check_bricks(vlj_sim* s) {
     s->last_check = s->steps;
   }
   return VLJ_OK;
+}
+
+// Whether a brick of the table stages more atoms than its layout's LDS holds
+// at the given cell occupancy: a handle whose first list build would cut
+// bricks runs the PARTS launches from the start. Otherwise it starts without
+// them (a brick outgrowing LDS later runs whole from global memory until the
+// periodic check turns them on).
+bool any_overflow(const vlj_sim* s, const std::vector<std::vector<uint32_t>>& occs) {
+  for (size_t b = 0; b < s->bricks.size(); b++) {
+    const Brick& g = s->bricks[b];
+    if (home_cells(g) < 2u) continue;
+    const Replica& R = s->rep[g.rep];
+    const std::vector<uint32_t>& occ = occs[g.rep];
+    const int nx = (int)R.cells[0], ny = (int)R.cells[1], nz = (int)R.cells[2];
+    const int x0 = (int)(g.cell0 % R.cells[0]), y0 = (int)((g.cell0 / R.cells[0]) % R.cells[1]);
+    const int z0 = (int)(g.cell0 / (R.cells[0] * R.cells[1]));
+    uint64_t total = 0;
+    for (int dz = 0; dz < (int)g.gz + 2; dz++)
+      for (int dy = 0; dy < (int)g.gy + 2; dy++)
+        for (int dx = 0; dx < (int)g.gx + 2; dx++) {
+          const int cx = (x0 - 1 + dx + nx) % nx, cy = (y0 - 1 + dy + ny) % ny, cz = (z0 - 1 + dz + nz) % nz;
+          total += occ[(size_t)(cz * ny + cy) * nx + cx];
+        }
+    if (total > (b < s->n_dense ? (uint64_t)kStageMax : (uint64_t)Sparse::kStageMax)) return true;
+  }
+  return false;
 }
 
 // Builds a handle over `replicas` systems.
@@ -885,11 +916,12 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   s->shifted = settings[0].shifted_force != 0;
   uint32_t atom_off = 0, cell_off = 0, scan_off = 0;
   uint64_t list_off = 0, prof_off = 0;
+  std::vector<std::vector<uint32_t>> occs(replicas);  // each replica's initial cell occupancy
   for (size_t r = 0; r < replicas; r++) {
     Replica& R = s->rep[r];
     std::string err;
     size_t n = (size_t)(offsets[r + 1] - offsets[r]);
-    std::vector<uint32_t> occ;
+    std::vector<uint32_t>& occ = occs[r];
     rc = derive(all + offsets[r], n, settings[r], R, s->host[r], occ, err);
     if (rc != VLJ_OK) {
       delete s;
@@ -1017,6 +1049,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   B.bricks = gb;
   B.n_bricks = (uint32_t)s->bricks.size();
   B.n_dense = s->n_dense;
+  B.parts_on = any_overflow(s, occs) ? 1u : 0u;
   B.n_atom_blocks = (uint32_t)s->atom_blocks.size();
   B.n_scan_blocks = (uint32_t)s->scan_blocks.size();
   B.n_sort_blocks = (uint32_t)s->sort_blocks.size();
@@ -2581,6 +2614,7 @@ int vlj_load(const char* path, vlj_sim** out) {
   s->bricks = bricks;
   s->n_dense = h.n_dense;
   B.n_dense = h.n_dense;
+  B.parts_on = std::any_of(a.cut.begin(), a.cut.end(), [](uint32_t np) { return np != 0u; }) ? 1u : 0u;
   s->rep_bricks.assign(R, 0u);
   for (const Brick& g : bricks) s->rep_bricks[g.rep]++;
   s->tuned = true;  // a loaded handle keeps the saved plan
