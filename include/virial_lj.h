@@ -360,6 +360,13 @@ int vlj_debug_params(const vlj_sim *sim, double out[6]);
  * in every thread of the process. */
 int vlj_debug_plan(uint32_t list_capacity, double stage_fill, double sparse_fill);
 
+/* Whether handles created afterwards run the PARTS launches, for tests (no
+ * counterpart in the reference): -1 (the rule) when their first list build
+ * would cut a brick, else only once the periodic check sees a brick outgrow
+ * LDS; 0 not from the start (a brick that outgrows LDS runs whole, reading
+ * positions through the cell tables); 1 from the start. */
+int vlj_debug_parts(int32_t mode);
+
 /* Workgroup-to-table mapping (no counterpart in the reference). With
  * blocks != 0: the table entry workgroup `block` works on -- workgroups go
  * round-robin to the 8 XCDs, and the table is dealt to them in chunks of 4

@@ -176,7 +176,7 @@ def test_bricks_cut_into_parts_stay_exact(oracle, lj, layout):
     count = int(round(0.8442 * np.prod(size)))
     pos, vel = oracle.lattice_box(count, size, 1.0, 5)
     st = lj.Settings(size, 2.5, 0.002, shifted_force=True, profile_bins=6)
-    with lj.debug_plan(stage_fill=100.0, sparse_fill=100.0 if layout == "sparse" else -1.0):
+    with lj.debug_plan(stage_fill=100.0, sparse_fill=100.0 if layout == "sparse" else -1.0, parts=True):
         sim = lj.LennardJones((pos, vel), st)
     ref = oracle.Sim(pos, vel, size, dt=0.002, shifted_force=True)
     ref.set_profile_bins(6)
@@ -198,6 +198,40 @@ def test_bricks_cut_into_parts_stay_exact(oracle, lj, layout):
     c = sim.debug_counters()
     assert c["bricks"] == 4 and c["stage_overflow_builds"] >= 4 * c["rebuilds"]
     assert c["global_cell_builds"] == 0
+    assert c["sparse_bricks"] == (4 if layout == "sparse" else 0)
+
+
+@pytest.mark.parametrize("layout", ["dense", "sparse"])
+def test_outgrown_bricks_run_whole_without_parts(oracle, lj, layout):
+    # the bricks of the test above, on a handle without the PARTS launches:
+    # each outgrown brick runs whole, every position read through the cell
+    # tables from global memory; lists, forces, measurements, profiles and
+    # the heat current stay exact
+    size = [24.0, 24.0, 12.0]
+    count = int(round(0.8442 * np.prod(size)))
+    pos, vel = oracle.lattice_box(count, size, 1.0, 5)
+    st = lj.Settings(size, 2.5, 0.002, shifted_force=True, profile_bins=6)
+    with lj.debug_plan(stage_fill=100.0, sparse_fill=100.0 if layout == "sparse" else -1.0, parts=False):
+        sim = lj.LennardJones((pos, vel), st)
+    ref = oracle.Sim(pos, vel, size, dt=0.002, shifted_force=True)
+    ref.set_profile_bins(6)
+    sim.record_heat_current(2)
+    ref.record_heat_current(2)
+    for steps in (1, 60):
+        sim.run(steps)
+        ref.run(steps)
+        m, r = sim.measure_now(), ref.measure()
+        assert list(m.raw) == list(r.raw)
+        np.testing.assert_array_equal(sim.debug_state()[0], ref.state()[0])
+        sim.sample_profile()
+        ref.sample_profile()
+        sim.sample_heat_current()
+        ref.sample_heat_current()
+    np.testing.assert_array_equal(sim.read_profile_raw()[0], ref.read_profile()[0])
+    np.testing.assert_array_equal(sim.read_heat_current_raw(), ref.read_heat_current())
+    c = sim.debug_counters()
+    assert c["bricks"] == 4 and c["stage_overflow_builds"] >= 4 * c["rebuilds"]
+    assert c["global_cell_builds"] >= 4 * c["rebuilds"]
     assert c["sparse_bricks"] == (4 if layout == "sparse" else 0)
 
 

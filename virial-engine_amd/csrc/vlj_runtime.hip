@@ -47,6 +47,7 @@ struct PlanOverrides {
   uint32_t list_capacity = 0;  // list entries per atom (at least 8)
   double stage_fill = 0.0;     // dense-layout bricks stage at most this share of kStageMax
   double sparse_fill = 0.0;    // a layer takes the sparse layout under this share of its stage max
+  int parts = -1;              // PARTS launches from the start: -1 when the first build would cut, 0 no, 1 yes
 };
 PlanOverrides g_plan;
 
@@ -1049,7 +1050,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   B.bricks = gb;
   B.n_bricks = (uint32_t)s->bricks.size();
   B.n_dense = s->n_dense;
-  B.parts_on = any_overflow(s, occs) ? 1u : 0u;
+  B.parts_on = g_plan.parts >= 0 ? (uint32_t)g_plan.parts : (any_overflow(s, occs) ? 1u : 0u);
   B.n_atom_blocks = (uint32_t)s->atom_blocks.size();
   B.n_scan_blocks = (uint32_t)s->scan_blocks.size();
   B.n_sort_blocks = (uint32_t)s->sort_blocks.size();
@@ -1836,6 +1837,12 @@ int vlj_debug_plan(uint32_t list_capacity, double stage_fill, double sparse_fill
   g_plan.list_capacity = list_capacity;
   g_plan.stage_fill = stage_fill;
   g_plan.sparse_fill = sparse_fill;
+  return VLJ_OK;
+}
+
+int vlj_debug_parts(int32_t mode) {
+  if (mode < -1 || mode > 1) return fail(nullptr, VLJ_E_INVALID, "bad argument to vlj_debug_parts");
+  g_plan.parts = mode;
   return VLJ_OK;
 }
 

@@ -103,7 +103,8 @@ extern "C" {
                             stream: *mut c_void) -> c_int;
 }
 // Not bound: vlj_debug_state, vlj_debug_cells, vlj_debug_lists, vlj_debug_grid,
-// vlj_debug_counters, vlj_debug_replica_counters, vlj_debug_params, vlj_debug_plan (planning overrides),
+// vlj_debug_counters, vlj_debug_replica_counters, vlj_debug_params, vlj_debug_plan and
+// vlj_debug_parts (planning overrides),
 // vlj_debug_swizzle (the XCD table mapping), vlj_debug_rcp and vlj_debug_rcp32
 // (the force reciprocals), vlj_copy_bandwidth, vlj_fp64_fma_rate and
 // vlj_timed_run are test and benchmark hooks with no counterpart in the

@@ -142,6 +142,7 @@ SIGNATURES = {
     "vlj_debug_counters": (C.c_int, [_P, _u64p]),
     "vlj_debug_replica_counters": (C.c_int, [_P, _sz, _u64p]),
     "vlj_debug_plan": (C.c_int, [_u32, _d, _d]),
+    "vlj_debug_parts": (C.c_int, [C.c_int32]),
     "vlj_debug_swizzle": (_u32, [_u32, _u32]),
     "vlj_copy_bandwidth": (C.c_int, [_sz, _u32, _dp]),
     "vlj_fp64_fma_rate": (C.c_int, [_u32, _dp]),

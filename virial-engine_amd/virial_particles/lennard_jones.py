@@ -801,17 +801,23 @@ def random_insertion_array(count: int, box, min_distance: float, temperature: fl
 class debug_plan:
     """Overrides the planning rules for handles created inside the block
     (tests only): list_capacity in place of the capacity rule, stage_fill and
-    sparse_fill in place of the brick planner's 0.9 (see vlj_debug_plan)."""
+    sparse_fill in place of the brick planner's 0.9 (see vlj_debug_plan);
+    parts True or False to run the PARTS launches from the start or not (see
+    vlj_debug_parts)."""
 
-    def __init__(self, list_capacity: int = 0, stage_fill: float = 0.0, sparse_fill: float = 0.0):
+    def __init__(self, list_capacity: int = 0, stage_fill: float = 0.0, sparse_fill: float = 0.0,
+                 parts: Optional[bool] = None):
         self.values = (list_capacity, stage_fill, sparse_fill)
+        self.parts = -1 if parts is None else int(bool(parts))
 
     def __enter__(self):
         _check(L.lib().vlj_debug_plan(*self.values))
+        _check(L.lib().vlj_debug_parts(self.parts))
         return self
 
     def __exit__(self, *exc):
         _check(L.lib().vlj_debug_plan(0, 0.0, 0.0))
+        _check(L.lib().vlj_debug_parts(-1))
 
 
 def debug_rcp(x) -> np.ndarray:
