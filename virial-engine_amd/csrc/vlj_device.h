@@ -187,7 +187,14 @@ struct Buffers {
   // this step's word (zeroed by the drift, or by the decision of the step
   // before, and counted by the decision) and the next step's
   uint32_t *rebuilding, *rebuilding_next;
+  // Drift-ahead form: kAheadSlots per replica of largest moved^2 since the
+  // list build, taken by the force launch into amax_next (atomicMax) for
+  // the next step, whose k_bin decides from them (bin_decides) in place of
+  // a k_decide launch; each force launch zeroes the set the bin just read.
+  unsigned long long *amax, *amax_next;
+  uint32_t bin_decides;
 };
+constexpr uint32_t kAheadSlots = 64;  // one wave's worth
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
 // XCD b % 8). Each table (atom blocks, sort blocks, bricks) is dealt to the

@@ -89,9 +89,9 @@ __device__ void decide_from(const Buffers& B, uint32_t r, double moved2, bool ad
 
 // One workgroup per replica: the largest moved^2 over the replica's drift
 // workgroups (bmax), then the decision. `advance` also counts the step and
-// sets each bmax it read back to zero (the drift-ahead force takes the
-// maximum into them); the first workgroup zeroes the next step's rebuild
-// word, which no kernel of this step reads.
+// zeroes the replica's drift-ahead maxima (the force launch of a drift-ahead
+// step takes them); the first workgroup zeroes the next step's rebuild word,
+// which no kernel of this step reads.
 __global__ void __launch_bounds__(kBlock) k_decide(Buffers B, int advance) {
   const uint32_t r = blockIdx.x;
   const Replica R = B.rep[r];
@@ -105,10 +105,11 @@ __global__ void __launch_bounds__(kBlock) k_decide(Buffers B, int advance) {
 #pragma unroll
       for (uint32_t j = 0; j < 16u; j++) v[j] = b0 + j * kBlock < end ? B.bmax[b0 + j * kBlock] : 0ull;
 #pragma unroll
-      for (uint32_t j = 0; j < 16u; j++) {
-        m = v[j] > m ? v[j] : m;
-        if (b0 + j * kBlock < end) B.bmax[b0 + j * kBlock] = 0ull;
-      }
+      for (uint32_t j = 0; j < 16u; j++) m = v[j] > m ? v[j] : m;
+    }
+    if (threadIdx.x < kAheadSlots) {
+      B.amax[(size_t)r * kAheadSlots + threadIdx.x] = 0ull;
+      B.amax_next[(size_t)r * kAheadSlots + threadIdx.x] = 0ull;
     }
     if (r == 0 && threadIdx.x == 0) *B.rebuilding_next = 0u;
   }
@@ -410,12 +411,33 @@ __device__ __forceinline__ uint32_t run_atomic_add(uint32_t* counter, uint32_t k
   return (uint32_t)__shfl(base, (int)leader, 64) + (lane - leader);
 }
 
+// With bin_decides (drift-ahead steps) every workgroup first takes the rebuild
+// decision of its replica from the maxima the last force launch left (the
+// same rule as k_decide's), and the replica's first atom block records it;
+// such steps come after a k_decide step, so `built` is already set and no
+// workgroup reads a word the recording one writes.
 __global__ void __launch_bounds__(kBlock) k_bin(Buffers B) {
-  if (*B.rebuilding == 0u) return;  // no replica rebuilds this step
   const uint32_t ab = dealt_index(blockIdx.x);
-  if (ab >= B.n_atom_blocks) return;
+  if (B.bin_decides) {
+    if (ab == 0 && threadIdx.x == 0) *B.rebuilding_next = 0u;
+    if (ab >= B.n_atom_blocks) return;
+    const uint32_t rep = B.atom_blocks[ab].rep;
+    __shared__ unsigned long long moved;
+    if (threadIdx.x < kAheadSlots) {
+      const unsigned long long v = wave_max_u64(B.amax[(size_t)rep * kAheadSlots + threadIdx.x]);
+      if (threadIdx.x == 0) moved = v;
+    }
+    __syncthreads();
+    const double moved2 = __longlong_as_double((long long)moved);
+    const bool rebuild = !B.ctl[rep].built || moved2 > B.rep[rep].half_skin2;
+    if (ab == B.rep[rep].atom_block_off && threadIdx.x == 0) decide_from(B, rep, moved2, true);
+    if (!rebuild) return;
+  } else {
+    if (*B.rebuilding == 0u) return;  // no replica rebuilds this step
+    if (ab >= B.n_atom_blocks) return;
+    if (!B.ctl[B.atom_blocks[ab].rep].rebuild) return;
+  }
   const Range blk = B.atom_blocks[ab];
-  if (!B.ctl[blk.rep].rebuild) return;
   const Replica R = B.rep[blk.rep];
   const uint32_t i = blk.first + threadIdx.x;
   const bool active = i < blk.end;
@@ -1605,8 +1627,9 @@ __device__ __forceinline__ PairAcc interactions(const Buffers& B, const Replica&
 // kick gives v(t + dt), and the next step's drift follows at once: its half
 // step and x(t + 2 dt) = x(t + dt) + v' dt go to the second position set
 // (x2), which no kernel of this step reads, and the largest moved^2 since
-// the list build into one of the replica's drift maxima (bmax, zeroed by
-// the decision that read them), so the next step needs no drift launch.
+// the list build into one of the replica's kAheadSlots maxima of the next
+// step (amax_next), so that step needs no drift launch; the set this step's
+// bin read (amax) is zeroed for the step after.
 template <int PREC, bool SF, class L>
 __device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, const Stage<L>& S, const Brick& g,
                                            uint32_t gb, bool staged, int kick) {
@@ -1648,8 +1671,10 @@ __device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, c
   }
   if (kick == 2) {
     moved = wave_max_u64(moved);
-    if ((threadIdx.x & 63u) == 0 && moved != 0ull && R.atom_blocks != 0u)
-      atomicMax(B.bmax + R.atom_block_off + gb % R.atom_blocks, moved);
+    if ((threadIdx.x & 63u) == 0 && moved != 0ull)
+      atomicMax(B.amax_next + (size_t)g.rep * kAheadSlots + gb % kAheadSlots, moved);
+    if (threadIdx.x == 0)
+      for (uint32_t k = gb; k < B.n_replicas * kAheadSlots; k += B.n_bricks) B.amax[k] = 0ull;
   }
 }
 
@@ -2484,11 +2509,6 @@ hipError_t launch_drift(const Buffers& B, bool langevin, bool keep_v, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_decide_step(const Buffers& B, hipStream_t s) {
-  if (!B.n_atom_blocks) return hipSuccess;
-  hipLaunchKernelGGL(k_decide, decide_grid(B), dim3(kBlock), 0, s, B, 1);
-  return hipGetLastError();
-}
 
 hipError_t launch_nh_half(const Buffers& B, bool scale, hipStream_t s) {
   if (!B.n_atom_blocks) return hipSuccess;

@@ -28,9 +28,6 @@ hipError_t launch_decide(const Buffers& B, hipStream_t s);
 // The drift and the step's rebuild decision; keep_v: positions only (the
 // drift-ahead form's first step).
 hipError_t launch_drift(const Buffers& B, bool langevin, bool keep_v, hipStream_t s);
-// The step's rebuild decision alone (the drift-ahead form, whose force
-// launch of the step before drifted the positions).
-hipError_t launch_decide_step(const Buffers& B, hipStream_t s);
 hipError_t launch_rebuild(const Buffers& B, hipStream_t s, const Side* side = nullptr);
 // The lists of every replica again from the positions of the last build
 // (xb), for a new brick table; the cut marks of the table must start at

@@ -389,6 +389,7 @@ void flip_positions(vlj_sim* s) {
   std::swap(B.y, B.y2);
   std::swap(B.z, B.z2);
   std::swap(B.rebuilding, B.rebuilding_next);
+  std::swap(B.amax, B.amax_next);
   s->orient ^= 1;
 }
 
@@ -397,11 +398,13 @@ int enqueue_step(vlj_sim* s, hipEvent_t* around_force = nullptr) {
   if (s->any_nose_hoover) HIP_TRY(s, launch_nh_half(s->B, false, s->stream));
   if (fused && s->ahead) {
     flip_positions(s);  // the positions the last force launch drifted
-    HIP_TRY(s, launch_decide_step(s->B, s->stream));
+    Buffers B = s->B;
+    B.bin_decides = 1u;  // the rebuild decision in k_bin, from the force launch's maxima
+    HIP_TRY(s, launch_rebuild(B, s->stream, &s->side));
   } else {
     HIP_TRY(s, launch_drift(s->B, s->any_langevin, fused, s->stream));
+    HIP_TRY(s, launch_rebuild(s->B, s->stream, &s->side));
   }
-  HIP_TRY(s, launch_rebuild(s->B, s->stream, &s->side));
   if (around_force) HIP_TRY(s, hipEventRecord(around_force[0], s->stream));
   HIP_TRY(s, launch_force(s->B, s->precision, s->shifted, fused ? 2 : 1, s->stream, &s->side));
   if (around_force) HIP_TRY(s, hipEventRecord(around_force[1], s->stream));
@@ -1018,6 +1021,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   A(&B.parts, s->bricks.size() * kHomeCells);
   A(&B.cut_groups, (s->bricks.size() + kCutGroup - 1) / kCutGroup);
   A(&B.bmax, s->atom_blocks.size());
+  A(&B.amax, 2 * replicas * kAheadSlots);
   A(&B.kepart, s->atom_blocks.size());
   A(&B.hexpart, s->atom_blocks.size() * kHexSums);
   A(&B.hexstate, replicas * kHexState);
@@ -1083,6 +1087,8 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
   if (e == hipSuccess) e = hipMemcpy(ob, s->sort_blocks.data(), s->sort_blocks.size() * sizeof(Range), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(gb, s->bricks.data(), s->bricks.size() * sizeof(Brick), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(B.bmax, 0, std::max<size_t>(s->atom_blocks.size(), 1) * 8);
+  if (e == hipSuccess) e = hipMemset(B.amax, 0, 2 * replicas * kAheadSlots * 8);
+  B.amax_next = B.amax + replicas * kAheadSlots;
   if (e == hipSuccess) e = hipMemset(B.cut, 0, std::max<size_t>(s->bricks.size(), 1) * 4);
   if (e == hipSuccess) e = hipMemset(B.cut_groups, 0, std::max<size_t>((s->bricks.size() + kCutGroup - 1) / kCutGroup, 1) * 4);
   // Zeroed once here; afterwards the sort kernel leaves them zero.
