@@ -25,12 +25,11 @@ constexpr int kBrickThreads = VLJ_BRICK_THREADS;  // threads of a brick workgrou
 static_assert(kBrickThreads % 64 == 0 && kBrickThreads <= 1024, "whole waves");
 constexpr int kScanCells = 1024;  // cells per scan block (256 threads x 4) of a dense replica
 constexpr int kScanCellsSparse = 4096;  // ... of a dilute one (256 threads x 16): fewer block sums
-constexpr int kSortCells = 4;     // cells per sort block of a dense replica (a wave64 per cell)
-// ... of a dense replica of fewer than kSortLargeReplica cells: a small
-// replica needs less parallelism, and fewer sort workgroups cost less on the
-// steps it does not rebuild (512-replica batch: +1.7% against 4 cells)
-constexpr int kSortCellsSmall = 16;
-constexpr uint32_t kSortLargeReplica = 16384;
+// Cells per sort block of a dense replica (a wave64 per cell, each wave
+// taking the block's cells in turn). The sort launch returns at once on the
+// steps without a rebuild, and fewer workgroups cost less there: against 4
+// cells, the 512-replica batch +1.7%, FCC-1M (13,718 blocks at 4) +1.2%.
+constexpr int kSortCells = 16;
 constexpr int kSortCellsSparse = 256;  // ... of a replica with fewer than kSortDense atoms per cell
 constexpr double kSortDense = 6.0;
 constexpr uint32_t kSmallCell = 8;  // a cell this small is ranked by one thread

@@ -961,8 +961,7 @@ int create(const vlj_particle* all, const uint64_t* offsets, const vlj_settings*
     for (uint32_t b = 0; b < R.total_cells; b += R.scan_cells)
       s->scan_blocks.push_back(
           {(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + R.scan_cells), 0});
-    const uint32_t per =
-        !dense ? kSortCellsSparse : (R.total_cells >= kSortLargeReplica ? kSortCells : kSortCellsSmall);
+    const uint32_t per = dense ? kSortCells : kSortCellsSparse;
     for (uint32_t b = 0; b < R.total_cells; b += per)
       s->sort_blocks.push_back({(uint32_t)r, cell_off + b, cell_off + std::min(R.total_cells, b + per), 0});
     atom_off += R.count;
