@@ -1495,10 +1495,14 @@ __device__ __forceinline__ void pair_force_f32(const Replica& R, double dx, doub
   fz = __builtin_fmaf((float)dz, fscal, fz);
 }
 
+// List octets (8 entries each) in flight per batch of the force loop. FCC-1M
+// (two-layer bricks, drift-ahead force launch; same box): 2 octets 129.5 us
+// per FP64 launch, 1 130.6, 3 132.8, 4 135.1, 8 170 (spills); the FP32 form
+// is the same at 2, 3 and 4.
 #ifndef VLJ_OCTET_BATCH
-#define VLJ_OCTET_BATCH 4
+#define VLJ_OCTET_BATCH 2
 #endif
-constexpr int kOctetBatch = VLJ_OCTET_BATCH;  // list octets (8 entries each) in flight per batch
+constexpr int kOctetBatch = VLJ_OCTET_BATCH;
 
 // Octet o of a list, o clamped to `last`, the list's own last octet (its
 // first when it is empty), so the load needs no branch and a clamped load
