@@ -761,9 +761,14 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
   uint32_t run = before + incl - n_t;
 #pragma unroll
   for (uint32_t h = 0; h < kCPT; h++) {
-    if (kCPT * t + h < ns) {
-      S.off[kCPT * t + h] = run;
-      S.base[kCPT * t + h] = st_c[h] - run;
+    const uint32_t c = kCPT * t + h;
+    if (c < ns) {
+      S.off[c] = run;
+      S.base[c] = st_c[h] - run;
+      // the coarse index: cell c holds staged atoms run .. run + n - 1 (those
+      // below kStageMax are indexed), known here without reading S.off back
+      const uint32_t end = run + n_c[h] < (uint32_t)L::kStageMax ? run + n_c[h] : (uint32_t)L::kStageMax;
+      for (uint32_t m = (run + kCoarse - 1u) / kCoarse; m * kCoarse < end; m++) S.coarse[m] = (uint16_t)c;
     }
     run += n_c[h];
   }
@@ -771,19 +776,6 @@ __device__ bool stage_brick(const Buffers& B, const Replica& R, const Brick& g, 
     S.off[ns] = before + incl;
     S.total = before + incl;
     S.ns = ns;
-  }
-  __syncthreads();
-  {
-    // the coarse index: cell c holds staged atoms off[c] .. off[c + 1] - 1
-    const uint32_t known = S.total < (uint32_t)L::kStageMax ? S.total : (uint32_t)L::kStageMax;
-#pragma unroll
-    for (uint32_t h = 0; h < kCPT; h++) {
-      const uint32_t c = kCPT * t + h;
-      if (c < ns) {
-        const uint32_t end = S.off[c + 1u] < known ? S.off[c + 1u] : known;
-        for (uint32_t m = (S.off[c] + kCoarse - 1u) / kCoarse; m * kCoarse < end; m++) S.coarse[m] = (uint16_t)c;
-      }
-    }
   }
   __syncthreads();
   if (t < 64u) {  // home atoms, home cell by home cell (hz, hy outer, hx inner): a scan over wave 0
