@@ -1636,18 +1636,34 @@ __device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, c
     const uint32_t i = at.slot;
     double xi, yi, zi;
     staged_pos(R, S, staged, at.e, B.x, B.y, B.z, xi, yi, zi);
-    const PairAcc G = interactions<PREC, SF, false>(B, R, S, staged, i, xi, yi, zi);
-    const double a[3] = {-24.0 * G.ax, -24.0 * G.ay, -24.0 * G.az};
     double* const A[3] = {B.ax, B.ay, B.az};
     double* const V[3] = {B.vx, B.vy, B.vz};
+    const double* const XB[3] = {B.xb, B.yb, B.zb};
+    // The atom's own v and a, loaded before its pairs so that the loads are
+    // in flight while they are summed (the 512-replica batch +3.3%, FCC-1M
+    // +0.9%) -- in the plain (truncated and shifted) forms; the
+    // shifted-force pair loops leave no room for them within 128 VGPRs.
+    constexpr bool kEarly = !SF;
+    double vo[3], ao[3] = {0.0, 0.0, 0.0};
+    auto own_state = [&] {
+#pragma unroll
+      for (int k = 0; k < 3; k++) vo[k] = V[k][i];
+      if (kick == 2) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) ao[k] = A[k][i];
+      }
+    };
+    if constexpr (kEarly) own_state();
+    const PairAcc G = interactions<PREC, SF, false>(B, R, S, staged, i, xi, yi, zi);
+    if constexpr (!kEarly) own_state();
+    const double a[3] = {-24.0 * G.ax, -24.0 * G.ay, -24.0 * G.az};
     if (kick == 2) {
       const double* const X[3] = {B.x, B.y, B.z};
       double* const X2[3] = {B.x2, B.y2, B.z2};
-      const double* const XB[3] = {B.xb, B.yb, B.zb};
       double d[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        const double half = __builtin_fma(A[k][i], R.half_dt, V[k][i]);  // the drift's half step
+        const double half = __builtin_fma(ao[k], R.half_dt, vo[k]);  // the drift's half step
         const double v = __builtin_fma(a[k], R.half_dt, half);
         A[k][i] = a[k];
         V[k][i] = v;
@@ -1661,7 +1677,7 @@ __device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, c
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         A[k][i] = a[k];
-        if (kick) V[k][i] = __builtin_fma(a[k], R.half_dt, V[k][i]);
+        if (kick) V[k][i] = __builtin_fma(a[k], R.half_dt, vo[k]);
       }
     }
   }
@@ -1677,7 +1693,7 @@ __device__ __forceinline__ void force_part(const Buffers& B, const Replica& R, c
 // One workgroup per brick, leaving bricks that are cut (see k_build) to the
 // PARTS launch that follows.
 template <class L, int PREC, bool SF, bool PARTS>
-__global__ void __launch_bounds__(L::kThreads) k_force(Buffers B, int kick) {
+__global__ void __launch_bounds__(L::kThreads, !PARTS && !L::kSparse ? 4 : 1) k_force(Buffers B, int kick) {
   __shared__ Stage<L> S;
   if constexpr (!PARTS) {
     const uint32_t gb = layout_first<L>(B) + dealt_index(blockIdx.x);
