@@ -1058,7 +1058,10 @@ __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
 // Measured on the 1M-atom liquid (ms per step, lists every ~10 steps):
 // piece 16 / 4 chunks 0.302, 64 / 4 0.298, 16 / 8 0.298, 32 / 10 0.290
 // (a dense cell's ~520 candidates all in registers, so each home atom's
-// position and list length are read once), 64 / 12 0.292.
+// position and list length are read once), 64 / 12 0.292. Again with
+// two-layer bricks and the drift-ahead step (atom-steps/s, one box):
+// 32 / 10 5.510e9 and 5.512e9, 32 / 9 5.523e9, 16 / 10 5.410e9, 12 / 10
+// 5.391e9, 32 / 12 4.79e9.
 #ifndef VLJ_BUILD_PIECE
 #define VLJ_BUILD_PIECE 32
 #endif
