@@ -693,7 +693,9 @@ __device__ __forceinline__ void image_shift(const Replica& R, uint32_t code, dou
 // ahead of the one summing pairs, so the CU spends less time with both
 // waiting on memory (FP64 force kernel at N = 1M: 165 -> 153 us; 1 and 3
 // alike). While the FP64 pair loop issues an octet's gathers: 1, below
-// staging.
+// staging. Again on the present step (FCC-1M, atom-steps/s, one box):
+// stage 2 / gathers 1 5.52e9-5.54e9, 1 / 1 5.53e9, 2 / 0 5.49e9, 2 / 2
+// 5.51e9, 3 / 1 5.41e9.
 #ifndef VLJ_STAGE_PRIO
 #define VLJ_STAGE_PRIO 2
 #endif
