@@ -1,8 +1,10 @@
 """Batched replicas on the GPU: replicas built like the batched benchmark
 (FCC sites for any count, random insertion at low density) run in one
-handle exactly as they do alone, and the dilute-gas check of the
-reference's tests/thermodynamics.rs:234-293 holds for each replica of a
-batch, its standard error coming from block means of the same run."""
+handle exactly as they do alone; an all-NVE batch (the drift-ahead step,
+dense and sparse bricks on two streams) matches the oracle replica by
+replica; and the dilute-gas check of the reference's
+tests/thermodynamics.rs:234-293 holds for each replica of a batch, its
+standard error coming from block means of the same run."""
 import math
 
 import numpy as np
@@ -91,3 +93,40 @@ def test_dilute_gas_virial_expansion_per_replica(lj):
         print(f"replica {r}: rho {rho} T {temp}: Z = {mean[r]:.5f} +/- {err[r]:.5f}, 1 + B2 rho = {predicted:.5f}")
         assert abs(mean[r] - 1.0) < 0.01
         assert abs(mean[r] - predicted) < 4.0 * max(err[r], 1e-4)
+
+
+def nve_replica(lj, n, rho, temp, seed):
+    parts, st = replica(lj, n, rho, temp, seed)
+    return parts, lj.Settings(list(st.box_size), 2.5, 0.005, skin=0.3)
+
+
+def test_nve_batch_on_both_layouts_against_the_oracle(oracle, lj):
+    # Every replica NVE, so the batch steps in the drift-ahead form; the
+    # dilute replicas' bricks take the sparse layout, whose launches run on
+    # the side stream beside the dense ones. Each replica against the
+    # oracle run alone, bit for bit, through list rebuilds.
+    specs = [nve_replica(lj, n, rho, t, s)
+             for n, rho, t, s in [(2048, 0.8442, 0.9, 21), (1500, 0.05, 1.5, 22), (3000, 0.7, 1.2, 23),
+                                  (900, 0.02, 2.0, 24)]]
+    batch = lj.Batch(specs)
+    c = batch.debug_counters()
+    assert 0 < c["sparse_bricks"] < c["bricks"], c
+    steps = 150
+    for chunk in (1, 17, steps - 18):  # an eager first step, a graph of 16 and one of 1, then the rest
+        batch.run(chunk)
+    ms = batch.measure_now()
+    allp = batch.read_particle_array()
+    off = 0
+    for r, (parts, st) in enumerate(specs):
+        ref = oracle.Sim(np.ascontiguousarray(parts["pos"]), np.ascontiguousarray(parts["vel"]),
+                         list(st.box_size), cutoff=2.5, skin=0.3, dt=0.005)
+        ref.run(steps)
+        m = ref.measure()
+        assert list(ms[r].raw) == list(m.raw), f"replica {r}"
+        assert ms[r].list_rebuilds == m.list_rebuilds > 1, f"replica {r}"
+        pos, vel = ref.read_particles()
+        n = len(parts)
+        np.testing.assert_array_equal(allp["pos"][off:off + n], pos)
+        np.testing.assert_array_equal(allp["vel"][off:off + n], vel)
+        off += n
+        ref.close()
