@@ -95,17 +95,19 @@ def test_dilute_gas_virial_expansion_per_replica(lj):
         assert abs(mean[r] - predicted) < 4.0 * max(err[r], 1e-4)
 
 
-def nve_replica(lj, n, rho, temp, seed):
+def nve_replica(lj, n, rho, temp, seed, precision, shifted):
     parts, st = replica(lj, n, rho, temp, seed)
-    return parts, lj.Settings(list(st.box_size), 2.5, 0.005, skin=0.3)
+    return parts, lj.Settings(list(st.box_size), 2.5, 0.005, skin=0.3, precision=precision, shifted_force=shifted)
 
 
-def test_nve_batch_on_both_layouts_against_the_oracle(oracle, lj):
+@pytest.mark.parametrize("precision,shifted", [(0, False), (1, True)])
+def test_nve_batch_on_both_layouts_against_the_oracle(oracle, lj, precision, shifted):
     # Every replica NVE, so the batch steps in the drift-ahead form; the
     # dilute replicas' bricks take the sparse layout, whose launches run on
     # the side stream beside the dense ones. Each replica against the
-    # oracle run alone, bit for bit, through list rebuilds.
-    specs = [nve_replica(lj, n, rho, t, s)
+    # oracle run alone, bit for bit, through list rebuilds: FP64 with the
+    # plain cutoff, and the FP32 form with the shifted force.
+    specs = [nve_replica(lj, n, rho, t, s, precision, shifted)
              for n, rho, t, s in [(2048, 0.8442, 0.9, 21), (1500, 0.05, 1.5, 22), (3000, 0.7, 1.2, 23),
                                   (900, 0.02, 2.0, 24)]]
     batch = lj.Batch(specs)
@@ -119,7 +121,8 @@ def test_nve_batch_on_both_layouts_against_the_oracle(oracle, lj):
     off = 0
     for r, (parts, st) in enumerate(specs):
         ref = oracle.Sim(np.ascontiguousarray(parts["pos"]), np.ascontiguousarray(parts["vel"]),
-                         list(st.box_size), cutoff=2.5, skin=0.3, dt=0.005)
+                         list(st.box_size), cutoff=2.5, skin=0.3, dt=0.005, shifted_force=shifted,
+                         precision=precision)
         ref.run(steps)
         m = ref.measure()
         assert list(ms[r].raw) == list(m.raw), f"replica {r}"
