@@ -1,8 +1,9 @@
 """Checkpoint and exact resume: a handle loaded from a state file steps on
 to the same bits as the handle that wrote it (NVE, Langevin with its
 step-keyed noise, Nose-Hoover, heat exchange and the heat current,
-profiles, a batch with bricks cut into parts), and both agree with the
-oracle run without interruption."""
+profiles, a batch with bricks cut into parts, an all-NVE batch saved
+between drift-ahead steps), and both agree with the oracle run without
+interruption."""
 import numpy as np
 import pytest
 
@@ -110,3 +111,37 @@ def test_resume_batch_with_cut_bricks(oracle, lj, tmp_path):
     np.testing.assert_array_equal(a["vel"], b["vel"])
     with pytest.raises(lj.LennardJonesError, match="Batch.load"):
         lj.LennardJones.load(path)
+
+
+def test_resume_nve_batch_with_cut_bricks(oracle, lj, tmp_path):
+    # every replica NVE: saved between drift-ahead steps (the next positions
+    # already formed in the second set), loaded, and both handles stepped
+    # on; each replica also against the oracle run without a break
+    parts = [sc_liquid(oracle, n_side=n) for n in (12, 9)]
+    specs = [((pos, vel), lj.Settings([box] * 3, 2.5, 0.005)) for pos, vel, box in parts]
+    with lj.debug_plan(stage_fill=100.0, parts=True):  # bricks that are cut into parts
+        batch = lj.Batch(specs)
+    batch.run(70)
+    assert batch.debug_counters()["stage_overflow_builds"] > 0
+    path = tmp_path / "nve_batch.state"
+    batch.save(path)
+    back = lj.Batch.load(path)
+    for steps in (1, 149):
+        batch.run(steps)
+        back.run(steps)
+    ms, bs = batch.measure_now(), back.measure_now()
+    a, b = batch.read_particle_array(), back.read_particle_array()
+    np.testing.assert_array_equal(a["pos"], b["pos"])
+    np.testing.assert_array_equal(a["vel"], b["vel"])
+    off = 0
+    for (pos, vel, box), m, k in zip(parts, ms, bs):
+        assert list(m.raw) == list(k.raw)
+        ref = oracle.Sim(pos, vel, [box] * 3)
+        ref.run(220)
+        assert list(m.raw) == list(ref.measure().raw)
+        rp, rv = ref.read_particles()
+        n = len(pos)
+        np.testing.assert_array_equal(a["pos"][off:off + n], rp)
+        np.testing.assert_array_equal(a["vel"][off:off + n], rv)
+        off += n
+        ref.close()
