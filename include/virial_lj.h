@@ -265,7 +265,10 @@ int vlj_batch_read_particles(vlj_batch *batch, vlj_particle *out, size_t count);
  * were cut into parts since the check before is planned again from the
  * occupancy of its last list build (its density rose since), held to 10%
  * less of the staging limit each time (down to 60%), with its lists rebuilt
- * for the new bricks. VLJ_E_STATE when vlj_debug_plan fixes the plan. */
+ * for the new bricks. VLJ_E_STATE when vlj_debug_plan fixes the plan.
+ * Trials time the state they run on: a handle started from a lattice that
+ * melts is best tuned again once it has (FCC 4 x 64^3: fills 0.9 and 1.0
+ * tie on the lattice, and 1.0 runs the liquid's force launch ~5% slower). */
 typedef struct {
     double fill;     /* the candidate's fill */
     double force_us; /* mean pair-force time per step */
